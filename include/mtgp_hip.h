@@ -1,0 +1,149 @@
+/*
+ * mtgp_hip.h -- GPU entry points of the MI355X MTGP library (libmtgp_amd.so).
+ *
+ * These are additions to the reference surface (mtgp32-fast.h,
+ * mtgp64-fast.h, mtgp*-fast-jump.h, which this library also exports): the
+ * reference computes every number on the host through inline genrand_*
+ * calls (mtgp32-fast.h:252-346) and its device code exists only as samples
+ * (cuda-sample/mtgp32-cuda.cu:171-297: one block per generator, state in
+ * LDS, state written back for the next call).  Here the same streams are
+ * produced by gfx950 kernels, bit-exact with the host functions.
+ *
+ * Conventions
+ *  - Functions return 0 (or a non-negative count) on success and -1 on
+ *    failure; mtgp_last_error() then describes the failure (thread-local).
+ *  - A "window" is a generator state as N words oldest first, N =
+ *    mexp/32+1 (uint32_t) or mexp/64+1 (uint64_t): the reference ring read
+ *    as array[(idx - N + 1 + j) & large_mask].
+ *  - Output kinds (MTGP_KIND_*) select genrand_uint32/uint64,
+ *    close1_open2, close_open, open_close or open_open; 4-byte outputs for
+ *    mtgp32, 8-byte outputs for mtgp64.
+ *  - `out` pointers of the device entry points are device memory; the
+ *    mtgp*_fill entry points also accept host memory.
+ *  - Streams are hipStream_t passed as void* (NULL = the null stream).
+ */
+#ifndef MTGP_HIP_H
+#define MTGP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "mtgp32-fast.h"
+#include "mtgp64-fast.h"
+
+#if defined(__cplusplus)
+extern "C" {
+#endif
+
+enum {
+    MTGP_KIND_INT = 0,          /* genrand_uint32 / genrand_uint64 */
+    MTGP_KIND_CLOSE1_OPEN2 = 1, /* [1, 2) */
+    MTGP_KIND_CLOSE_OPEN = 2,   /* [0, 1) */
+    MTGP_KIND_OPEN_CLOSE = 3,   /* (0, 1] */
+    MTGP_KIND_OPEN_OPEN = 4     /* (0, 1) */
+};
+
+/* ------------------------------------------------------------ plumbing */
+const char *mtgp_last_error(void);
+/* Makes `device` current for this library (default: HIP's current device). */
+int mtgp_set_device(int device);
+/* Compute units and HBM bytes of the current device. */
+int mtgp_device_info(int *compute_units, size_t *hbm_bytes);
+void *mtgp_dev_malloc(size_t bytes);
+int mtgp_dev_free(void *ptr);
+int mtgp_memcpy_htod(void *dst, const void *src, size_t bytes);
+int mtgp_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int mtgp_synchronize(void);
+/* Streams and events for callers without their own HIP code (timing). */
+void *mtgp_stream_create(void);
+int mtgp_stream_destroy(void *stream);
+int mtgp_stream_synchronize(void *stream);
+void *mtgp_event_create(void);
+int mtgp_event_destroy(void *event);
+int mtgp_event_record(void *event, void *stream);
+int mtgp_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ------------------------------------ bulk generation of host generators
+ * Continue the stream of a generator created by mtgp32_init* (or
+ * mtgp64_init*) on the GPU: writes n numbers of `kind` to out (device or
+ * host memory) and leaves `mtgp` exactly where n calls of the matching
+ * genrand_* would have left it (same window; the ring is rewritten with
+ * idx = size - 1).  Long requests are split across the GPU by jump-ahead.
+ * Replaces a loop of mtgp32_genrand_* (mtgp32-fast.h:252-346). */
+int mtgp32_fill(mtgp32_fast_t *mtgp32, int kind, void *out, size_t n);
+int mtgp64_fill(mtgp64_fast_t *mtgp64, int kind, void *out, size_t n);
+
+/* ------------------------------------------------- jump polynomials
+ * NTL-free replacements for calc_characteristic (mtgp32-calc-poly.cpp:
+ * 36-45) and calc_jump (mtgp-calc-jump.hpp:135-162).  Results are hex
+ * strings in the polytostring format accepted by mtgp*_fast_jump; the
+ * return value is the string length (buffer too small: -1).  The
+ * characteristic polynomial is checked against params->poly_sha1.
+ * `steps` is a little-endian array of nwords 64-bit words. */
+int mtgp32_charpoly_hex(const mtgp32_params_fast_t *params, char *buf, size_t cap);
+int mtgp64_charpoly_hex(const mtgp64_params_fast_t *params, char *buf, size_t cap);
+int mtgp32_jump_hex(const mtgp32_params_fast_t *params, const uint64_t *steps, int nwords,
+                    char *buf, size_t cap);
+int mtgp64_jump_hex(const mtgp64_params_fast_t *params, const uint64_t *steps, int nwords,
+                    char *buf, size_t cap);
+
+/* -------------------------------------------- device generator sets
+ * A set of generators whose states stay in device memory.  Generator g
+ * uses parameter set index[g] of the table for (bits[g], mexp[g]). */
+typedef struct mtgp_genset mtgp_genset_t;
+
+mtgp_genset_t *mtgp_genset_create(int count, const int *bits, const int *mexp, const int *index);
+void mtgp_genset_destroy(mtgp_genset_t *set);
+int mtgp_genset_count(const mtgp_genset_t *set);
+/* Words in generator g's window (N). */
+int mtgp_genset_state_words(const mtgp_genset_t *set, int g);
+/* mtgp32_init / mtgp64_init with seeds[g] (low 32 bits for mtgp32). */
+int mtgp_genset_init_seed(mtgp_genset_t *set, const uint64_t *seeds);
+/* mtgp*_init_by_array: generator g uses keys[offsets[g] .. +lengths[g]]
+ * (low 32 bits of each key word for mtgp32). */
+int mtgp_genset_init_array(mtgp_genset_t *set, const uint64_t *keys, const int *offsets,
+                           const int *lengths);
+/* Host <-> device copy of one window (uint32_t or uint64_t words). */
+int mtgp_genset_set_state(mtgp_genset_t *set, int g, const void *window);
+int mtgp_genset_get_state(mtgp_genset_t *set, int g, void *window);
+/* Replace generator g's state by its jump by a polynomial in hex (as
+ * mtgp*_fast_jump does), on the device. */
+int mtgp_genset_jump(mtgp_genset_t *set, int g, const char *jump_hex);
+/* Make generator g of `set` the substream `i` of a base state: the base
+ * window jumped i times by `jump_hex` (host-composed into one polynomial).
+ * Used for the reference's substream scheme (mtgp32-jump.cl:697-731). */
+int mtgp_genset_substreams(mtgp_genset_t *set, const void *base_window, const char *jump_hex);
+
+/* One request: `count` numbers of `kind` from generator `gen`, after first
+ * skipping `skip` numbers, written to device memory `out`.  Requests on the
+ * same generator are consecutive parts of its stream, in array order. */
+typedef struct mtgp_request {
+    int32_t gen;
+    int32_t kind;
+    uint64_t count;
+    uint64_t skip;
+    void *out;
+} mtgp_request_t;
+
+/* Batched generation.  A plan fixes the requests, splits each generator's
+ * stream into segments of about `segment` numbers (0 = chosen from the
+ * size of the batch) whose start states are reached by device jump-ahead,
+ * and stores the work list in device memory.  mtgp_plan_launch enqueues
+ * one persistent kernel launch per (word size, state size class) on
+ * `stream`; it can be called again (the same numbers are produced again,
+ * from the same states) and captured in a hipGraph.  mtgp_plan_commit
+ * makes the states after the plan the generators' current states. */
+typedef struct mtgp_plan mtgp_plan_t;
+mtgp_plan_t *mtgp_plan_create(mtgp_genset_t *set, const mtgp_request_t *reqs, int nreq,
+                              uint64_t segment);
+int mtgp_plan_launch(mtgp_plan_t *plan, void *stream);
+int mtgp_plan_commit(mtgp_plan_t *plan);
+void mtgp_plan_destroy(mtgp_plan_t *plan);
+/* Work items, kernel launches, and host planning time of a plan. */
+int mtgp_plan_stats(const mtgp_plan_t *plan, int *items, int *launches, double *plan_ms);
+/* plan + launch + commit + synchronize. */
+int mtgp_genset_generate(mtgp_genset_t *set, const mtgp_request_t *reqs, int nreq, void *stream);
+
+#if defined(__cplusplus)
+}
+#endif
+#endif /* MTGP_HIP_H */

@@ -1,0 +1,428 @@
+"""MI355X-native MTGP: Python front end over libmtgp_amd.so (ctypes).
+
+The library exports the reference C surface (mtgp32-fast.h, mtgp64-fast.h,
+mtgp32/64-fast-jump.h) plus the GPU entry points of include/mtgp_hip.h.
+This module mirrors the reference's operator interface in Python:
+
+* ``Generator32`` / ``Generator64`` wrap ``mtgp32_fast_t`` / ``mtgp64_fast_t``
+  (init by seed, by array, by string; the genrand family; ``fast_jump``),
+  with ``fill()`` producing any number of outputs on the GPU;
+* ``charpoly_hex`` / ``jump_hex`` are the NTL-free replacements of
+  calc_characteristic / calc_jump;
+* ``GenSet`` holds many device-resident generators and runs batched
+  requests (``Plan``) as persistent kernel launches.
+
+Device buffers are ``DeviceBuffer`` objects (the library's own hipMalloc),
+so no other GPU framework needs to be loaded into the process.  If the
+library has not been built, importing this package raises ImportError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmtgp_amd.so")
+
+KIND_INT, KIND_CLOSE1_OPEN2, KIND_CLOSE_OPEN, KIND_OPEN_CLOSE, KIND_OPEN_OPEN = range(5)
+KINDS = {"uint": 0, "close1_open2": 1, "close_open": 2, "open_close": 3, "open_open": 4}
+MEXPS = {32: (11213, 23209, 44497), 64: (23209, 44497, 110503)}
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libmtgp_amd.so is not built: run `python -m mtgp_amd.build`")
+
+_lib = C.CDLL(LIB_PATH, mode=os.RTLD_LOCAL)
+
+
+class MTGPError(RuntimeError):
+    pass
+
+
+def last_error():
+    return _lib.mtgp_last_error().decode()
+
+
+def _check(rc, what):
+    if rc is None or (isinstance(rc, int) and rc < 0):
+        raise MTGPError("%s: %s" % (what, last_error()))
+    return rc
+
+
+# --------------------------------------------------------------- structs
+class Params32(C.Structure):
+    """mtgp32_params_fast_t (mtgp32-fast.h:47-58)."""
+    _fields_ = [("mexp", C.c_int), ("pos", C.c_int), ("sh1", C.c_int), ("sh2", C.c_int),
+                ("tbl", C.c_uint32 * 16), ("tmp_tbl", C.c_uint32 * 16),
+                ("flt_tmp_tbl", C.c_uint32 * 16), ("mask", C.c_uint32),
+                ("poly_sha1", C.c_ubyte * 21)]
+
+
+class Params64(C.Structure):
+    """mtgp64_params_fast_t (mtgp64-fast.h:46-57)."""
+    _fields_ = [("mexp", C.c_int), ("pos", C.c_int), ("sh1", C.c_int), ("sh2", C.c_int),
+                ("tbl", C.c_uint64 * 16), ("tmp_tbl", C.c_uint64 * 16),
+                ("dbl_tmp_tbl", C.c_uint64 * 16), ("mask", C.c_uint64),
+                ("poly_sha1", C.c_ubyte * 21)]
+
+
+class Status(C.Structure):
+    """Header of mtgp32_status_fast_t / mtgp64_status_fast_t (array follows)."""
+    _fields_ = [("idx", C.c_int), ("size", C.c_int), ("large_size", C.c_int),
+                ("large_mask", C.c_int)]
+
+
+class Fast32(C.Structure):
+    _fields_ = [("params", Params32), ("status", C.POINTER(Status))]
+
+
+class Fast64(C.Structure):
+    _fields_ = [("params", Params64), ("status", C.POINTER(Status))]
+
+
+class Request(C.Structure):
+    """mtgp_request_t."""
+    _fields_ = [("gen", C.c_int32), ("kind", C.c_int32), ("count", C.c_uint64),
+                ("skip", C.c_uint64), ("out", C.c_void_p)]
+
+
+def _sig(name, res, *args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+vp, i32, u64, sz, cp = C.c_void_p, C.c_int, C.c_uint64, C.c_size_t, C.c_char_p
+_sig("mtgp_last_error", C.c_char_p)
+_sig("mtgp_set_device", i32, i32)
+_sig("mtgp_device_info", i32, C.POINTER(C.c_int), C.POINTER(C.c_size_t))
+_sig("mtgp_dev_malloc", vp, sz)
+_sig("mtgp_dev_free", i32, vp)
+_sig("mtgp_memcpy_htod", i32, vp, vp, sz)
+_sig("mtgp_memcpy_dtoh", i32, vp, vp, sz)
+_sig("mtgp_synchronize", i32)
+_sig("mtgp_stream_create", vp)
+_sig("mtgp_stream_destroy", i32, vp)
+_sig("mtgp_stream_synchronize", i32, vp)
+_sig("mtgp_event_create", vp)
+_sig("mtgp_event_destroy", i32, vp)
+_sig("mtgp_event_record", i32, vp, vp)
+_sig("mtgp_event_elapsed_ms", i32, vp, vp, C.POINTER(C.c_float))
+for b in (32, 64):
+    P, F = (Params32, Fast32) if b == 32 else (Params64, Fast64)
+    W = C.c_uint32 if b == 32 else C.c_uint64
+    _sig("mtgp%d_init" % b, i32, C.POINTER(F), C.POINTER(P), W)
+    _sig("mtgp%d_init_state" % b, None, C.POINTER(W), C.POINTER(P), W)
+    _sig("mtgp%d_init_by_array" % b, i32, C.POINTER(F), C.POINTER(P), C.POINTER(W), i32)
+    _sig("mtgp%d_init_by_str" % b, i32, C.POINTER(F), C.POINTER(P), cp)
+    _sig("mtgp%d_free" % b, None, C.POINTER(F))
+    _sig("mtgp%d_fast_jump" % b, None, C.POINTER(F), cp)
+    _sig("mtgp%d_fill" % b, i32, C.POINTER(F), i32, vp, sz)
+    _sig("mtgp%d_charpoly_hex" % b, i32, C.POINTER(P), cp, sz)
+    _sig("mtgp%d_jump_hex" % b, i32, C.POINTER(P), C.POINTER(C.c_uint64), i32, cp, sz)
+_sig("mtgp_genset_create", vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("mtgp_genset_destroy", None, vp)
+_sig("mtgp_genset_count", i32, vp)
+_sig("mtgp_genset_state_words", i32, vp, i32)
+_sig("mtgp_genset_init_seed", i32, vp, C.POINTER(C.c_uint64))
+_sig("mtgp_genset_init_array", i32, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("mtgp_genset_set_state", i32, vp, i32, vp)
+_sig("mtgp_genset_get_state", i32, vp, i32, vp)
+_sig("mtgp_genset_jump", i32, vp, i32, cp)
+_sig("mtgp_genset_substreams", i32, vp, vp, cp)
+_sig("mtgp_plan_create", vp, vp, C.POINTER(Request), i32, u64)
+_sig("mtgp_plan_launch", i32, vp, vp)
+_sig("mtgp_plan_commit", i32, vp)
+_sig("mtgp_plan_destroy", None, vp)
+_sig("mtgp_plan_stats", i32, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double))
+_sig("mtgp_genset_generate", i32, vp, C.POINTER(Request), i32, vp)
+
+lib = _lib
+
+
+def params(bits, mexp, index):
+    """The reference parameter record mtgp{bits}_params_fast_{mexp}[index]."""
+    if mexp not in MEXPS.get(bits, ()) or not 0 <= index < 128:
+        raise ValueError("no parameter set mtgp%d-%d #%d" % (bits, mexp, index))
+    P = Params32 if bits == 32 else Params64
+    table = (P * 128).in_dll(_lib, "mtgp%d_params_fast_%d" % (bits, mexp))
+    return table[index]
+
+
+def _wdtype(bits):
+    return np.uint32 if bits == 32 else np.uint64
+
+
+def out_dtype(bits, kind):
+    if kind in (KIND_INT, KIND_CLOSE1_OPEN2):
+        return np.uint32 if bits == 32 else np.uint64
+    return np.float32 if bits == 32 else np.float64
+
+
+# ------------------------------------------------------------ device memory
+def device_info():
+    cus, hbm = C.c_int(), C.c_size_t()
+    _check(_lib.mtgp_device_info(C.byref(cus), C.byref(hbm)), "mtgp_device_info")
+    return cus.value, hbm.value
+
+
+class DeviceBuffer:
+    """A device allocation made by the library (hipMalloc)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = _lib.mtgp_dev_malloc(max(1, self.nbytes))
+        if not self.ptr:
+            raise MTGPError("mtgp_dev_malloc(%d): %s" % (nbytes, last_error()))
+
+    def free(self):
+        if self.ptr:
+            _lib.mtgp_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def address(self, offset=0):
+        return self.ptr + int(offset)
+
+    def to_host(self, dtype, count=None, offset=0):
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else int(count)
+        a = np.empty(count, dtype)
+        if count:
+            _check(_lib.mtgp_memcpy_dtoh(a.ctypes.data, self.ptr + offset, a.nbytes), "memcpy dtoh")
+        return a
+
+    def from_host(self, a, offset=0):
+        a = np.ascontiguousarray(a)
+        _check(_lib.mtgp_memcpy_htod(self.ptr + offset, a.ctypes.data, a.nbytes), "memcpy htod")
+
+
+def synchronize():
+    _check(_lib.mtgp_synchronize(), "mtgp_synchronize")
+
+
+class Stream:
+    def __init__(self):
+        self.handle = _lib.mtgp_stream_create()
+        if not self.handle:
+            raise MTGPError(last_error())
+
+    def synchronize(self):
+        _check(_lib.mtgp_stream_synchronize(self.handle), "stream sync")
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.mtgp_stream_destroy(self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        self.handle = _lib.mtgp_event_create()
+        if not self.handle:
+            raise MTGPError(last_error())
+
+    def record(self, stream=None):
+        _check(_lib.mtgp_event_record(self.handle, stream.handle if stream else None), "event record")
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        _check(_lib.mtgp_event_elapsed_ms(self.handle, end.handle, C.byref(ms)), "elapsed")
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.mtgp_event_destroy(self.handle)
+            self.handle = None
+
+
+# ------------------------------------------------------ host-side generators
+class _Generator:
+    bits = 32
+
+    def __init__(self, mexp, index, seed=None, array=None, string=None):
+        self.p = params(self.bits, mexp, index)
+        F = Fast32 if self.bits == 32 else Fast64
+        self.g = F()
+        W = C.c_uint32 if self.bits == 32 else C.c_uint64
+        if array is not None:
+            key = (W * len(array))(*[int(v) for v in array])
+            rc = getattr(_lib, "mtgp%d_init_by_array" % self.bits)(C.byref(self.g), C.byref(self.p), key, len(array))
+        elif string is not None:
+            rc = getattr(_lib, "mtgp%d_init_by_str" % self.bits)(C.byref(self.g), C.byref(self.p), string)
+        else:
+            rc = getattr(_lib, "mtgp%d_init" % self.bits)(C.byref(self.g), C.byref(self.p), int(seed or 0))
+        if rc:
+            raise MemoryError("mtgp%d_init failed" % self.bits)
+
+    def __del__(self):
+        g = getattr(self, "g", None)
+        if g is not None and g.status:
+            getattr(_lib, "mtgp%d_free" % self.bits)(C.byref(g))
+
+    @property
+    def n(self):
+        return self.g.status.contents.size
+
+    def window(self):
+        """Canonical state window (oldest word first)."""
+        st = self.g.status.contents
+        W = C.c_uint32 if self.bits == 32 else C.c_uint64
+        arr = (W * st.large_size).from_address(C.addressof(st) + C.sizeof(Status))
+        ring = np.frombuffer(arr, dtype=_wdtype(self.bits)).copy()
+        idx = (st.idx - st.size + 1 + np.arange(st.size) + st.large_size) & st.large_mask
+        return ring[idx]
+
+    def fill(self, kind, n, out=None):
+        """n numbers of `kind` on the GPU.  out: None (returns a numpy array),
+        a numpy array (filled in place), or a DeviceBuffer."""
+        kind = KINDS.get(kind, kind)
+        dt = out_dtype(self.bits, kind)
+        if out is None:
+            out = np.empty(n, dt)
+        if isinstance(out, DeviceBuffer) or hasattr(out, "ptr"):
+            ptr = out.ptr
+        else:
+            if out.dtype != dt or out.size < n or not out.flags.c_contiguous:
+                raise ValueError("output array must be contiguous %s of >= %d" % (dt, n))
+            ptr = out.ctypes.data
+        _check(getattr(_lib, "mtgp%d_fill" % self.bits)(C.byref(self.g), kind, ptr, n), "fill")
+        return out
+
+    def fast_jump(self, hexstr):
+        """mtgp*_fast_jump on the GPU (mtgp32-fast-jump.c:51-79)."""
+        getattr(_lib, "mtgp%d_fast_jump" % self.bits)(C.byref(self.g), hexstr.encode())
+        err = last_error()
+        if err:
+            raise MTGPError("fast_jump: " + err)
+
+
+class Generator32(_Generator):
+    bits = 32
+
+
+class Generator64(_Generator):
+    bits = 64
+
+
+def charpoly_hex(bits, mexp, index):
+    p = params(bits, mexp, index)
+    buf = C.create_string_buffer(mexp // 4 + 8)
+    _check(getattr(_lib, "mtgp%d_charpoly_hex" % bits)(C.byref(p), buf, len(buf)), "charpoly")
+    return buf.value.decode()
+
+
+def jump_hex(bits, mexp, index, steps):
+    """x^steps mod P as a jump string (steps: Python int >= 0)."""
+    p = params(bits, mexp, index)
+    words = max(1, (int(steps).bit_length() + 63) // 64)
+    e = (C.c_uint64 * words)(*[(int(steps) >> (64 * i)) & (2**64 - 1) for i in range(words)])
+    buf = C.create_string_buffer(mexp // 4 + 8)
+    _check(getattr(_lib, "mtgp%d_jump_hex" % bits)(C.byref(p), e, words, buf, len(buf)), "jump_hex")
+    return buf.value.decode()
+
+
+# ------------------------------------------------------------ device sets
+class GenSet:
+    """Device-resident generators (mtgp_genset_t)."""
+
+    def __init__(self, specs):
+        """specs: list of (bits, mexp, index)."""
+        self.specs = [tuple(int(v) for v in s) for s in specs]
+        n = len(self.specs)
+        b = (C.c_int * n)(*[s[0] for s in self.specs])
+        m = (C.c_int * n)(*[s[1] for s in self.specs])
+        i = (C.c_int * n)(*[s[2] for s in self.specs])
+        self.h = _lib.mtgp_genset_create(n, b, m, i)
+        if not self.h:
+            raise MTGPError("genset_create: " + last_error())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.mtgp_genset_destroy(self.h)
+            self.h = None
+
+    def __len__(self):
+        return len(self.specs)
+
+    def init_seed(self, seeds):
+        s = (C.c_uint64 * len(self))(*[int(v) & (2**64 - 1) for v in seeds])
+        _check(_lib.mtgp_genset_init_seed(self.h, s), "init_seed")
+
+    def init_array(self, keys):
+        flat, offs, lens = [], [], []
+        for k in keys:
+            offs.append(len(flat))
+            lens.append(len(k))
+            flat.extend(int(v) & (2**64 - 1) for v in k)
+        n = len(self)
+        _check(_lib.mtgp_genset_init_array(self.h, (C.c_uint64 * max(1, len(flat)))(*flat),
+                                           (C.c_int * n)(*offs), (C.c_int * n)(*lens)), "init_array")
+
+    def state(self, g):
+        n = _lib.mtgp_genset_state_words(self.h, g)
+        w = np.empty(n, _wdtype(self.specs[g][0]))
+        _check(_lib.mtgp_genset_get_state(self.h, g, w.ctypes.data), "get_state")
+        return w
+
+    def set_state(self, g, window):
+        w = np.ascontiguousarray(window, _wdtype(self.specs[g][0]))
+        _check(_lib.mtgp_genset_set_state(self.h, g, w.ctypes.data), "set_state")
+
+    def jump(self, g, hexstr):
+        _check(_lib.mtgp_genset_jump(self.h, g, hexstr.encode()), "jump")
+
+    def substreams(self, base_window, hexstr):
+        w = np.ascontiguousarray(base_window, _wdtype(self.specs[0][0]))
+        _check(_lib.mtgp_genset_substreams(self.h, w.ctypes.data, hexstr.encode()), "substreams")
+
+    def plan(self, requests, segment=0):
+        return Plan(self, requests, segment)
+
+    def generate(self, requests, stream=None):
+        reqs = _requests(requests)
+        _check(_lib.mtgp_genset_generate(self.h, reqs, len(requests), stream.handle if stream else None),
+               "generate")
+
+
+def _requests(requests):
+    arr = (Request * max(1, len(requests)))()
+    for k, r in enumerate(requests):
+        gen, kind, count, skip, out = r
+        arr[k].gen, arr[k].kind, arr[k].count, arr[k].skip = gen, KINDS.get(kind, kind), count, skip
+        arr[k].out = out.ptr if isinstance(out, DeviceBuffer) else out
+    return arr
+
+
+class Plan:
+    """A batch of requests compiled to device work items (mtgp_plan_t).
+    requests: list of (gen, kind, count, skip, out) with out a DeviceBuffer
+    or a device address."""
+
+    def __init__(self, genset, requests, segment=0):
+        self.genset = genset
+        self._reqs = _requests(requests)
+        self.h = _lib.mtgp_plan_create(genset.h, self._reqs, len(requests), int(segment))
+        if not self.h:
+            raise MTGPError("plan_create: " + last_error())
+
+    def launch(self, stream=None):
+        _check(_lib.mtgp_plan_launch(self.h, stream.handle if stream else None), "plan_launch")
+
+    def commit(self):
+        _check(_lib.mtgp_plan_commit(self.h), "plan_commit")
+
+    def stats(self):
+        it, ln, ms = C.c_int(), C.c_int(), C.c_double()
+        _lib.mtgp_plan_stats(self.h, C.byref(it), C.byref(ln), C.byref(ms))
+        return {"items": it.value, "launches": ln.value, "plan_ms": ms.value}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.mtgp_plan_destroy(self.h)
+            self.h = None

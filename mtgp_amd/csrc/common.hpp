@@ -1,0 +1,62 @@
+// common.hpp -- definitions shared by the host library and the HIP kernels.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+namespace mtgp {
+
+// One unit of device work: continue (or first jump, then continue) the
+// stream of one generator for `count` numbers of one output kind.
+//
+//  - state_in  : canonical window (N words, oldest first; 64-bit words for
+//                MTGP64) the stream continues from.
+//  - poly      : optional jump polynomial, 32 coefficients per uint32_t word
+//                (bit b of word i = coefficient of x^(32i+b)).  When set the
+//                window is first replaced by sum_k c_k T^k window.
+//  - out       : output address (device memory), 4 or 8 bytes per number.
+//  - state_out : optional; receives the window after the last number.
+struct alignas(16) Item {
+    uint64_t out;
+    uint64_t count;
+    uint64_t state_in;
+    uint64_t state_out;
+    uint64_t poly;
+    int32_t poly_words;  // number of uint32_t coefficient words
+    int32_t param;       // index into the device parameter table
+    int32_t kind;        // 0 int, 1 [1,2), 2 [0,1), 3 (0,1], 4 (0,1)
+    int32_t pad;
+    uint64_t pad2;
+};
+static_assert(sizeof(Item) == 64, "Item layout");
+
+// Expanded parameter record as the kernels read it.  For MTGP64 the
+// tables hold the high 32 bits of each entry (all low halves are zero).
+struct alignas(16) DevParam {
+    int32_t bits, mexp, n, pos, sh1, sh2;
+    uint32_t mask_lo, mask_hi;
+    uint32_t tbl[16];
+    uint32_t tmp[16];
+    uint32_t flt[16];
+};
+static_assert(sizeof(DevParam) == 224, "DevParam layout");
+
+// Global parameter-set ids: 6 tables x 128 sets, in the order
+// mtgp32 11213, 23209, 44497, mtgp64 23209, 44497, 110503.
+constexpr int kSetsPerTable = 128;
+constexpr int kNumTables = 6;
+constexpr int kNumSets = kSetsPerTable * kNumTables;
+
+int table_of(int bits, int mexp);                 // -1 if unknown
+inline int global_set_id(int bits, int mexp, int index) {
+    int t = table_of(bits, mexp);
+    return (t < 0 || index < 0 || index >= kSetsPerTable) ? -1 : t * kSetsPerTable + index;
+}
+const DevParam &host_param(int gid);              // expanded record
+inline int state_words(int bits, int mexp) { return mexp / bits + 1; }
+
+// Thread-local error text (mtgp_last_error).
+void set_error(const std::string &msg);
+const char *last_error();
+
+}  // namespace mtgp

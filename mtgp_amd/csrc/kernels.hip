@@ -1,0 +1,632 @@
+// kernels.hip -- MTGP32 / MTGP64 generation and jump-ahead kernels (gfx950).
+//
+// Replaces the device samples (cuda-sample/mtgp32-cuda.cu:171-297,
+// mtgp64-cuda.cu:208-392, openCL-sample/mtgp32.cl:167-399, and the device
+// jump of openCL-sample/mtgp32-jump.cl:630-731) with one persistent kernel
+// per word size that works through a list of Items (common.hpp).
+//
+// Design (one workgroup = one wave64 = one Item at a time):
+//  * The state window of N words lives in a linear LDS buffer.  A step
+//    appends 256 words: lane l computes words 4l..4l+3, which are mutually
+//    independent because every parameter set has N - pos >= 256.  The step
+//    reads x1/x2 as one aligned 128-bit LDS load plus one word, y/t as five
+//    words, looks the 4-bit indices up in 16-entry tables held in LDS, and
+//    writes its outputs as one 128-bit global store per lane.
+//  * Instead of a masked ring, the buffer is linear and the live window is
+//    copied back to the start every K steps, so every LDS address of a step
+//    is a per-lane base plus an immediate offset.
+//  * Jump-ahead (optional per Item): with c the jump polynomial, the new
+//    window is new[j] = XOR_{k : c_k = 1} w[k + j] over the generator's own
+//    sequence w.  The kernel streams w through the same LDS buffer K steps
+//    at a time and XOR-accumulates into registers (lane l owns j = l + 64m),
+//    scanning c's set bits with scalar instructions.  This is the same
+//    state as the reference's mtgp32_fast_jump (mtgp32-fast-jump.c:51-79).
+//  * Work distribution: workgroup g starts with Item g, then takes further
+//    Items from an atomic counter; the last workgroup to finish resets the
+//    counter pair, so launches need no memset and are graph-capturable.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace mtgp {
+namespace {
+
+// Device-memory pointers carry address space 1 so that accesses through
+// addresses taken from Items compile to global_* (not flat_*) instructions.
+#define GMEM __attribute__((address_space(1)))
+typedef GMEM uint32_t gu32;
+typedef GMEM uint64_t gu64;
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+typedef GMEM v4u gu4;
+typedef GMEM v2u gu2;
+
+constexpr int kWave = 64;
+constexpr int kStep = 256;  // words per wave step (4 per lane)
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Single-wave workgroup: orders this wave's LDS accesses across steps.
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+__device__ __forceinline__ uint32_t temper_index(uint32_t t) {
+    t ^= t >> 16;
+    t ^= t >> 8;
+    return t & 15;
+}
+
+// ----------------------------------------------------------------- MTGP32
+
+struct P32 {
+    int n, pos, sh1, sh2;
+    uint32_t mask;
+};
+
+// mtgp32-fast.h:149-159
+__device__ __forceinline__ uint32_t rec32(const P32 &p, const uint32_t *tbl, uint32_t x1, uint32_t x2,
+                                          uint32_t y) {
+    uint32_t x = (x1 & p.mask) ^ x2;
+    x ^= x << p.sh1;
+    y = x ^ (y >> p.sh2);
+    return y ^ tbl[y & 15];
+}
+
+// Output word for each kind (mtgp32-fast.h:192-346).  All float operations
+// are exact: x - 1 and 2 - x for x in [1, 2) are representable.
+template <int KIND>
+__device__ __forceinline__ uint32_t out32(const uint32_t *tab, uint32_t r, uint32_t t) {
+    const uint32_t ti = temper_index(t);
+    if (KIND == 0)
+        return r ^ tab[16 + ti];
+    const uint32_t u = (r >> 9) ^ tab[32 + ti];
+    if (KIND == 1)
+        return u;
+    if (KIND == 2)
+        return __float_as_uint(__uint_as_float(u) - 1.0f);
+    if (KIND == 3)
+        return __float_as_uint(2.0f - __uint_as_float(u));
+    return __float_as_uint(__uint_as_float(u | 1u) - 1.0f);
+}
+
+// One 256-word step from window base b (b % 4 == 0).  With OUT, writes the
+// m (<= 256) leading outputs to dst (16-byte aligned).
+template <int KIND, bool OUT>
+__device__ __forceinline__ void step32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b,
+                                       gu32 *dst, uint32_t m) {
+    const int l = threadIdx.x;
+    const uint32_t *xp = buf + b + 4 * l;
+    const uint4 xa = *reinterpret_cast<const uint4 *>(xp);
+    const uint32_t x4 = xp[4];
+    const uint32_t *yp = xp + p.pos - 1;
+    const uint32_t t0 = yp[0], t1 = yp[1], t2 = yp[2], t3 = yp[3], t4 = yp[4];
+    const uint32_t r0 = rec32(p, tab, xa.x, xa.y, t1);
+    const uint32_t r1 = rec32(p, tab, xa.y, xa.z, t2);
+    const uint32_t r2 = rec32(p, tab, xa.z, xa.w, t3);
+    const uint32_t r3 = rec32(p, tab, xa.w, x4, t4);
+    uint32_t *wp = buf + b + p.n + 4 * l;
+    wp[0] = r0;
+    wp[1] = r1;
+    wp[2] = r2;
+    wp[3] = r3;
+    if (OUT) {
+        uint4 o;
+        o.x = out32<KIND>(tab, r0, t0);
+        o.y = out32<KIND>(tab, r1, t1);
+        o.z = out32<KIND>(tab, r2, t2);
+        o.w = out32<KIND>(tab, r3, t3);
+        const uint32_t q = 4 * l;
+        if (q + 4 <= m) {
+            *reinterpret_cast<gu4 *>(dst + q) = v4u{o.x, o.y, o.z, o.w};
+        } else if (q < m) {
+            dst[q] = o.x;
+            if (q + 1 < m)
+                dst[q + 1] = o.y;
+            if (q + 2 < m)
+                dst[q + 2] = o.z;
+        }
+    }
+}
+
+// Copies the window [b, b + n) to [0, n) through registers (the ranges may
+// overlap).
+template <int NJ>
+__device__ __forceinline__ void relocate32(uint32_t *buf, int b, int n) {
+    const int l = threadIdx.x;
+    uint32_t v[NJ];
+#pragma unroll
+    for (int m = 0; m < NJ; m++)
+        v[m] = (l + kWave * m < n) ? buf[b + l + kWave * m] : 0u;
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < NJ; m++)
+        if (l + kWave * m < n)
+            buf[l + kWave * m] = v[m];
+    wave_sync();
+}
+
+// Window after jumping: acc[m] = new[l + 64m] (see file header).
+template <int NJ, int K, int NBUF>
+__device__ void jump32(uint32_t *buf, const uint32_t *tab, const P32 &p, const gu32 *__restrict__ in,
+                       const gu32 *__restrict__ poly, int poly_words, uint32_t (&acc)[NJ]) {
+    const int l = threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < NJ; m++)
+        acc[m] = 0;
+    for (int j = l; j < p.n; j += kWave)
+        buf[j] = in[j];
+    wave_sync();
+    const int total = poly_words * 32;
+    for (int a = 0; a < total;) {
+        const int left = total - a;
+        const int steps = left >= K * kStep ? K : (left + kStep - 1) / kStep;
+        for (int s = 0; s < steps; s++) {
+            step32<0, false>(buf, tab, p, s * kStep, nullptr, 0);
+            wave_sync();
+        }
+        // buf now holds w[a .. a + n + 256*steps); fold positions
+        // [a, a + 256*steps) with set coefficients into acc.
+        const int qend = (a + steps * kStep < total ? a + steps * kStep : total) / 32;
+        for (int q = a / 32; q < qend; q++) {
+            uint32_t cw = uniform(poly[q]);
+            while (cw) {
+                const int bit = __builtin_ctz(cw);
+                cw &= cw - 1;
+                const uint32_t *src = buf + (32 * q + bit - a) + l;
+#pragma unroll
+                for (int m = 0; m < NJ; m++)
+                    acc[m] ^= src[kWave * m];
+            }
+        }
+        a += steps * kStep;
+        if (a < total)
+            relocate32<NJ>(buf, steps * kStep, p.n);
+    }
+    wave_sync();
+}
+
+template <int KIND, int NJ, int NBUF>
+__device__ void generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int o, int h,
+                           gu32 *__restrict__ out, uint64_t count, gu32 *__restrict__ state_out) {
+    const int l = threadIdx.x;
+    // Head: h < 4 words, one per lane, so that the main loop stores are
+    // 16-byte aligned; the window base becomes o + h == 0 mod 4.
+    if (h > 0) {
+        if (l < h) {
+            const int q = o + l;
+            const uint32_t r = rec32(p, tab, buf[q], buf[q + 1], buf[q + p.pos]);
+            buf[q + p.n] = r;
+            out[l] = out32<KIND>(tab, r, buf[q + p.pos - 1]);
+        }
+        wave_sync();
+    }
+    int b = o + h;
+    uint64_t done = (uint64_t)h;
+    while (done < count) {
+        if (b + p.n + kStep > NBUF) {
+            relocate32<NJ>(buf, b, p.n);
+            b = 0;
+        }
+        const uint64_t left = count - done;
+        const uint32_t m = left < (uint64_t)kStep ? (uint32_t)left : (uint32_t)kStep;
+        step32<KIND, true>(buf, tab, p, b, out + done, m);
+        wave_sync();
+        b += (int)m;
+        done += m;
+    }
+    if (state_out)
+        for (int j = l; j < p.n; j += kWave)
+            state_out[j] = buf[b + j];
+}
+
+template <int NMAX, int K>
+__global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict__ items, int n_items,
+                                                          const DevParam *__restrict__ params,
+                                                          uint32_t *__restrict__ queue) {
+    constexpr int NJ = (NMAX + kWave - 1) / kWave;
+    constexpr int NBUF = K * kStep + kWave * NJ + 8;
+    __shared__ __attribute__((aligned(16))) uint32_t buf[NBUF];
+    __shared__ uint32_t tab[48];
+    const int l = threadIdx.x;
+    uint32_t it = blockIdx.x;
+    while (it < (uint32_t)n_items) {
+        const Item &I = items[it];
+        const DevParam &D = params[I.param];
+        P32 p;
+        p.n = D.n;
+        p.pos = D.pos;
+        p.sh1 = D.sh1;
+        p.sh2 = D.sh2;
+        p.mask = D.mask_lo;
+        if (l < 48)
+            tab[l] = D.tbl[l];  // tbl, tmp, flt are contiguous
+        const uint64_t count = I.count;
+        gu32 *out = reinterpret_cast<gu32 *>(I.out);
+        gu32 *state_out = reinterpret_cast<gu32 *>(I.state_out);
+        const gu32 *state_in = reinterpret_cast<const gu32 *>(I.state_in);
+        uint32_t h = (uint32_t)((16u - (uint32_t)(I.out & 15)) & 15) >> 2;
+        if ((uint64_t)h > count)
+            h = (uint32_t)count;
+        const int o = (int)((4u - h) & 3u);
+        wave_sync();
+        if (I.poly) {
+            uint32_t acc[NJ];
+            jump32<NJ, K, NBUF>(buf, tab, p, state_in, reinterpret_cast<const gu32 *>(I.poly),
+                                I.poly_words, acc);
+#pragma unroll
+            for (int m = 0; m < NJ; m++)
+                if (l + kWave * m < p.n)
+                    buf[o + l + kWave * m] = acc[m];
+        } else {
+            for (int j = l; j < p.n; j += kWave)
+                buf[o + j] = state_in[j];
+        }
+        wave_sync();
+        switch (I.kind) {
+        case 0: generate32<0, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        case 1: generate32<1, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        case 2: generate32<2, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        case 3: generate32<3, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        default: generate32<4, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        }
+        wave_sync();
+        uint32_t next = 0;
+        if (l == 0)
+            next = atomicAdd(&queue[0], 1u);
+        it = gridDim.x + uniform(__shfl(next, 0));
+    }
+    if (l == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
+// ----------------------------------------------------------------- MTGP64
+// Words are kept as separate high/low 32-bit planes in LDS.
+
+struct P64 {
+    int n, pos, sh1, sh2;
+    uint32_t mask_hi, mask_lo;
+};
+
+// mtgp64-fast.h:148-169: (rh, rl) = rec(x1, x2, y); only the high half of
+// a table entry is non-zero.
+__device__ __forceinline__ void rec64(const P64 &p, const uint32_t *tbl, uint32_t x1h, uint32_t x1l,
+                                      uint32_t x2h, uint32_t x2l, uint32_t yh, uint32_t yl, uint32_t &rh,
+                                      uint32_t &rl) {
+    uint32_t xh = (x1h & p.mask_hi) ^ x2h;
+    uint32_t xl = (x1l & p.mask_lo) ^ x2l;
+    xh ^= xh << p.sh1;
+    xl ^= xl << p.sh1;
+    const uint32_t nh = xl ^ (yh >> p.sh2);
+    const uint32_t nl = xh ^ (yl >> p.sh2);
+    rh = nh ^ tbl[nl & 15];
+    rl = nl;
+}
+
+// mtgp64-fast.h:202-356.  The double tables' low halves are zero.
+template <int KIND>
+__device__ __forceinline__ void out64(const uint32_t *tab, uint32_t rh, uint32_t rl, uint32_t tl, uint32_t &oh,
+                                      uint32_t &ol) {
+    const uint32_t ti = temper_index(tl);
+    if (KIND == 0) {
+        oh = rh ^ tab[16 + ti];
+        ol = rl;
+        return;
+    }
+    uint32_t lo = (rl >> 12) | (rh << 20);
+    const uint32_t hi = (rh >> 12) ^ tab[32 + ti];
+    if (KIND == 4)
+        lo |= 1u;
+    if (KIND == 1) {
+        oh = hi;
+        ol = lo;
+        return;
+    }
+    const double d = __hiloint2double((int)hi, (int)lo);
+    double v;
+    if (KIND == 3)
+        v = 2.0 - d;
+    else
+        v = d - 1.0;
+    oh = (uint32_t)__double2hiint(v);
+    ol = (uint32_t)__double2loint(v);
+}
+
+template <int KIND, bool OUT>
+__device__ __forceinline__ void step64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b,
+                                       gu32 *dst, uint32_t m) {
+    const int l = threadIdx.x;
+    const int q = b + 4 * l;
+    const uint4 xh = *reinterpret_cast<const uint4 *>(bh + q);
+    const uint4 xl = *reinterpret_cast<const uint4 *>(bl + q);
+    const uint32_t x4h = bh[q + 4], x4l = bl[q + 4];
+    const int yq = q + p.pos - 1;
+    const uint32_t th0 = bh[yq], th1 = bh[yq + 1], th2 = bh[yq + 2], th3 = bh[yq + 3], th4 = bh[yq + 4];
+    const uint32_t tl0 = bl[yq], tl1 = bl[yq + 1], tl2 = bl[yq + 2], tl3 = bl[yq + 3], tl4 = bl[yq + 4];
+    uint32_t rh0, rl0, rh1, rl1, rh2, rl2, rh3, rl3;
+    rec64(p, tab, xh.x, xl.x, xh.y, xl.y, th1, tl1, rh0, rl0);
+    rec64(p, tab, xh.y, xl.y, xh.z, xl.z, th2, tl2, rh1, rl1);
+    rec64(p, tab, xh.z, xl.z, xh.w, xl.w, th3, tl3, rh2, rl2);
+    rec64(p, tab, xh.w, xl.w, x4h, x4l, th4, tl4, rh3, rl3);
+    (void)th0;
+    const int w = q + p.n;
+    bh[w] = rh0;
+    bh[w + 1] = rh1;
+    bh[w + 2] = rh2;
+    bh[w + 3] = rh3;
+    bl[w] = rl0;
+    bl[w + 1] = rl1;
+    bl[w + 2] = rl2;
+    bl[w + 3] = rl3;
+    if (OUT) {
+        uint4 o0, o1;
+        out64<KIND>(tab, rh0, rl0, tl0, o0.y, o0.x);
+        out64<KIND>(tab, rh1, rl1, tl1, o0.w, o0.z);
+        out64<KIND>(tab, rh2, rl2, tl2, o1.y, o1.x);
+        out64<KIND>(tab, rh3, rl3, tl3, o1.w, o1.z);
+        const uint32_t e = 4 * l;  // first output element of this lane
+        gu4 *d4 = reinterpret_cast<gu4 *>(dst + 2 * e);
+        if (e + 4 <= m) {
+            d4[0] = v4u{o0.x, o0.y, o0.z, o0.w};
+            d4[1] = v4u{o1.x, o1.y, o1.z, o1.w};
+        } else if (e < m) {
+            gu2 *d2 = reinterpret_cast<gu2 *>(dst + 2 * e);
+            d2[0] = v2u{o0.x, o0.y};
+            if (e + 1 < m)
+                d2[1] = v2u{o0.z, o0.w};
+            if (e + 2 < m)
+                d2[2] = v2u{o1.x, o1.y};
+        }
+    }
+}
+
+template <int NJ>
+__device__ __forceinline__ void relocate64(uint32_t *bh, uint32_t *bl, int b, int n) {
+    const int l = threadIdx.x;
+    uint32_t vh[NJ], vl[NJ];
+#pragma unroll
+    for (int m = 0; m < NJ; m++) {
+        const bool ok = l + kWave * m < n;
+        vh[m] = ok ? bh[b + l + kWave * m] : 0u;
+        vl[m] = ok ? bl[b + l + kWave * m] : 0u;
+    }
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < NJ; m++)
+        if (l + kWave * m < n) {
+            bh[l + kWave * m] = vh[m];
+            bl[l + kWave * m] = vl[m];
+        }
+    wave_sync();
+}
+
+template <int NJ, int K, int NBUF>
+__device__ void jump64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p,
+                       const gu64 *__restrict__ in, const gu32 *__restrict__ poly, int poly_words,
+                       uint32_t (&ah)[NJ], uint32_t (&al)[NJ]) {
+    const int l = threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < NJ; m++)
+        ah[m] = al[m] = 0;
+    for (int j = l; j < p.n; j += kWave) {
+        const uint64_t v = in[j];
+        bh[j] = (uint32_t)(v >> 32);
+        bl[j] = (uint32_t)v;
+    }
+    wave_sync();
+    const int total = poly_words * 32;
+    for (int a = 0; a < total;) {
+        const int left = total - a;
+        const int steps = left >= K * kStep ? K : (left + kStep - 1) / kStep;
+        for (int s = 0; s < steps; s++) {
+            step64<0, false>(bh, bl, tab, p, s * kStep, nullptr, 0);
+            wave_sync();
+        }
+        const int qend = (a + steps * kStep < total ? a + steps * kStep : total) / 32;
+        for (int q = a / 32; q < qend; q++) {
+            uint32_t cw = uniform(poly[q]);
+            while (cw) {
+                const int bit = __builtin_ctz(cw);
+                cw &= cw - 1;
+                const int off = (32 * q + bit - a) + l;
+#pragma unroll
+                for (int m = 0; m < NJ; m++) {
+                    ah[m] ^= bh[off + kWave * m];
+                    al[m] ^= bl[off + kWave * m];
+                }
+            }
+        }
+        a += steps * kStep;
+        if (a < total)
+            relocate64<NJ>(bh, bl, steps * kStep, p.n);
+    }
+    wave_sync();
+}
+
+template <int KIND, int NJ, int NBUF>
+__device__ void generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int o, int h,
+                           gu32 *__restrict__ out, uint64_t count, gu64 *__restrict__ state_out) {
+    const int l = threadIdx.x;
+    if (h > 0) {
+        if (l < h) {
+            const int q = o + l;
+            uint32_t rh, rl, oh, ol;
+            rec64(p, tab, bh[q], bl[q], bh[q + 1], bl[q + 1], bh[q + p.pos], bl[q + p.pos], rh, rl);
+            bh[q + p.n] = rh;
+            bl[q + p.n] = rl;
+            out64<KIND>(tab, rh, rl, bl[q + p.pos - 1], oh, ol);
+            reinterpret_cast<gu2 *>(out)[l] = v2u{ol, oh};
+        }
+        wave_sync();
+    }
+    int b = o + h;
+    uint64_t done = (uint64_t)h;
+    while (done < count) {
+        if (b + p.n + kStep > NBUF) {
+            relocate64<NJ>(bh, bl, b, p.n);
+            b = 0;
+        }
+        const uint64_t left = count - done;
+        const uint32_t m = left < (uint64_t)kStep ? (uint32_t)left : (uint32_t)kStep;
+        step64<KIND, true>(bh, bl, tab, p, b, out + 2 * done, m);
+        wave_sync();
+        b += (int)m;
+        done += m;
+    }
+    if (state_out)
+        for (int j = l; j < p.n; j += kWave)
+            state_out[j] = ((uint64_t)bh[b + j] << 32) | bl[b + j];
+}
+
+template <int NMAX, int K>
+__global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict__ items, int n_items,
+                                                          const DevParam *__restrict__ params,
+                                                          uint32_t *__restrict__ queue) {
+    constexpr int NJ = (NMAX + kWave - 1) / kWave;
+    constexpr int NBUF = K * kStep + kWave * NJ + 8;
+    __shared__ __attribute__((aligned(16))) uint32_t bh[NBUF];
+    __shared__ __attribute__((aligned(16))) uint32_t bl[NBUF];
+    __shared__ uint32_t tab[48];
+    const int l = threadIdx.x;
+    uint32_t it = blockIdx.x;
+    while (it < (uint32_t)n_items) {
+        const Item &I = items[it];
+        const DevParam &D = params[I.param];
+        P64 p;
+        p.n = D.n;
+        p.pos = D.pos;
+        p.sh1 = D.sh1;
+        p.sh2 = D.sh2;
+        p.mask_hi = D.mask_hi;
+        p.mask_lo = D.mask_lo;
+        if (l < 48)
+            tab[l] = D.tbl[l];
+        const uint64_t count = I.count;
+        gu32 *out = reinterpret_cast<gu32 *>(I.out);
+        gu64 *state_out = reinterpret_cast<gu64 *>(I.state_out);
+        const gu64 *state_in = reinterpret_cast<const gu64 *>(I.state_in);
+        uint32_t h = (uint32_t)((I.out & 15) ? 1u : 0u);  // 8-byte outputs
+        if ((uint64_t)h > count)
+            h = (uint32_t)count;
+        const int o = (int)((4u - h) & 3u);
+        wave_sync();
+        if (I.poly) {
+            uint32_t ah[NJ], al[NJ];
+            jump64<NJ, K, NBUF>(bh, bl, tab, p, state_in, reinterpret_cast<const gu32 *>(I.poly),
+                                I.poly_words, ah, al);
+#pragma unroll
+            for (int m = 0; m < NJ; m++)
+                if (l + kWave * m < p.n) {
+                    bh[o + l + kWave * m] = ah[m];
+                    bl[o + l + kWave * m] = al[m];
+                }
+        } else {
+            for (int j = l; j < p.n; j += kWave) {
+                const uint64_t v = state_in[j];
+                bh[o + j] = (uint32_t)(v >> 32);
+                bl[o + j] = (uint32_t)v;
+            }
+        }
+        wave_sync();
+        switch (I.kind) {
+        case 0: generate64<0, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        case 1: generate64<1, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        case 2: generate64<2, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        case 3: generate64<3, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        default: generate64<4, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        }
+        wave_sync();
+        uint32_t next = 0;
+        if (l == 0)
+            next = atomicAdd(&queue[0], 1u);
+        it = gridDim.x + uniform(__shfl(next, 0));
+    }
+    if (l == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
+// Steps per LDS relocation, by class.
+constexpr int kK = 3;
+
+template <int B, int NMAX>
+struct KernelOf;
+template <int NMAX>
+struct KernelOf<32, NMAX> {
+    static constexpr auto fn = mtgp32_items_kernel<NMAX, kK>;
+    static constexpr int lds = 4 * ((kK * kStep + kWave * ((NMAX + 63) / 64) + 8) + 48);
+};
+template <int NMAX>
+struct KernelOf<64, NMAX> {
+    static constexpr auto fn = mtgp64_items_kernel<NMAX, kK>;
+    static constexpr int lds = 4 * (2 * (kK * kStep + kWave * ((NMAX + 63) / 64) + 8) + 48);
+};
+
+template <class KO>
+hipError_t launch_one(const Item *items, int n_items, const DevParam *params, uint32_t *queue, int grid,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(KO::fn, dim3(grid), dim3(kWave), 0, s, items, n_items, params, queue);
+    return hipGetLastError();
+}
+
+template <class KO>
+const void *fn_of() {
+    return reinterpret_cast<const void *>(KO::fn);
+}
+
+}  // namespace
+
+// Class c of each word size: N <= 351/726/1391 (32-bit), 363/696/1727 (64).
+int kernel_class(int bits, int n) {
+    if (bits == 32)
+        return n <= 351 ? 0 : n <= 726 ? 1 : n <= 1391 ? 2 : -1;
+    return n <= 363 ? 0 : n <= 696 ? 1 : n <= 1727 ? 2 : -1;
+}
+
+const void *kernel_fn(int bits, int cls) {
+    if (bits == 32) {
+        switch (cls) {
+        case 0: return fn_of<KernelOf<32, 351>>();
+        case 1: return fn_of<KernelOf<32, 726>>();
+        default: return fn_of<KernelOf<32, 1391>>();
+        }
+    }
+    switch (cls) {
+    case 0: return fn_of<KernelOf<64, 363>>();
+    case 1: return fn_of<KernelOf<64, 696>>();
+    default: return fn_of<KernelOf<64, 1727>>();
+    }
+}
+
+int kernel_lds_bytes(int bits, int cls) {
+    if (bits == 32)
+        return cls == 0 ? KernelOf<32, 351>::lds : cls == 1 ? KernelOf<32, 726>::lds : KernelOf<32, 1391>::lds;
+    return cls == 0 ? KernelOf<64, 363>::lds : cls == 1 ? KernelOf<64, 696>::lds : KernelOf<64, 1727>::lds;
+}
+
+hipError_t launch_items(int bits, int cls, const Item *items, int n_items, const DevParam *params,
+                        uint32_t *queue, int grid, hipStream_t s) {
+    if (bits == 32) {
+        switch (cls) {
+        case 0: return launch_one<KernelOf<32, 351>>(items, n_items, params, queue, grid, s);
+        case 1: return launch_one<KernelOf<32, 726>>(items, n_items, params, queue, grid, s);
+        default: return launch_one<KernelOf<32, 1391>>(items, n_items, params, queue, grid, s);
+        }
+    }
+    switch (cls) {
+    case 0: return launch_one<KernelOf<64, 363>>(items, n_items, params, queue, grid, s);
+    case 1: return launch_one<KernelOf<64, 696>>(items, n_items, params, queue, grid, s);
+    default: return launch_one<KernelOf<64, 1727>>(items, n_items, params, queue, grid, s);
+    }
+}
+
+}  // namespace mtgp

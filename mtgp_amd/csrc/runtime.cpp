@@ -1,0 +1,1151 @@
+// runtime.cpp -- device context, jump-polynomial service, batched planner
+// and the C entry points of mtgp_hip.h / mtgp*-fast-jump.h.
+#include <hip/hip_runtime_api.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "mtgp32-fast.h"
+#include "mtgp32-fast-jump.h"
+#include "mtgp64-fast.h"
+#include "mtgp64-fast-jump.h"
+#include "mtgp_hip.h"
+
+#include "common.hpp"
+#include "gf2x.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace mtgp {
+
+Error::Error(const std::string &m) : std::runtime_error(m) {}
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw Error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------- context
+
+namespace {
+std::mutex g_ctx_mu;
+std::map<int, std::unique_ptr<Context>> g_ctx;
+int g_device = -1;
+}  // namespace
+
+Context &context() {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    int dev = g_device;
+    if (dev < 0) {
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (count <= 0)
+            throw Error("no GPU visible to HIP");
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+    }
+    auto it = g_ctx.find(dev);
+    if (it != g_ctx.end()) {
+        hip_check(hipSetDevice(dev), "hipSetDevice");
+        return *it->second;
+    }
+    hip_check(hipSetDevice(dev), "hipSetDevice");
+    std::unique_ptr<Context> c(new Context());
+    c->device = dev;
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    c->cus = prop.multiProcessorCount;
+    c->hbm_bytes = prop.totalGlobalMem;
+    c->arch = prop.gcnArchName;
+    // Parameter table: the 768 reference sets, then room for custom sets.
+    c->param_capacity = kNumSets + 256;
+    hip_check(hipMalloc((void **)&c->d_params, sizeof(DevParam) * (size_t)c->param_capacity), "hipMalloc params");
+    std::vector<DevParam> host(kNumSets);
+    for (int k = 0; k < kNumSets; k++)
+        host[(size_t)k] = host_param(k);
+    hip_check(hipMemcpy(c->d_params, host.data(), sizeof(DevParam) * host.size(), hipMemcpyHostToDevice),
+              "hipMemcpy params");
+    c->n_params = kNumSets;
+    for (int b = 0; b < 2; b++)
+        for (int cls = 0; cls < 3; cls++) {
+            int nb = 0;
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_fn(b ? 64 : 32, cls), 64, 0),
+                      "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+            c->blocks_per_cu[b][cls] = nb > 0 ? nb : 1;
+        }
+    Context &ref = *c;
+    g_ctx[dev] = std::move(c);
+    return ref;
+}
+
+void select_device(int dev) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    g_device = dev;
+}
+
+// Parameter slot of an arbitrary (host) parameter record: a table set when
+// it equals one, else a custom slot uploaded on first use.
+int Context::slot_of(const DevParam &rec, const unsigned char *sha1) {
+    for (int k = 0; k < kNumSets; k++)
+        if (memcmp(&host_param(k), &rec, sizeof(DevParam)) == 0)
+            return k;
+    std::lock_guard<std::mutex> lk(mu);
+    for (size_t k = 0; k < customs.size(); k++)
+        if (memcmp(&customs[k].rec, &rec, sizeof(DevParam)) == 0)
+            return kNumSets + (int)k;
+    if (n_params >= param_capacity)
+        throw Error("too many custom parameter sets");
+    hip_check(hipMemcpy(d_params + n_params, &rec, sizeof(DevParam), hipMemcpyHostToDevice), "hipMemcpy param");
+    Custom cst;
+    cst.rec = rec;
+    memcpy(cst.sha1, sha1, 20);
+    customs.push_back(cst);
+    return n_params++;
+}
+
+const DevParam &Context::param(int slot) const {
+    return slot < kNumSets ? host_param(slot) : customs[(size_t)(slot - kNumSets)].rec;
+}
+
+void Context::sha1_of(int slot, unsigned char out[20]) const {
+    if (slot < kNumSets) {
+        const int t = slot / kSetsPerTable, i = slot % kSetsPerTable;
+        static mtgp32_params_fast_t *const t32[3] = {mtgp32_params_fast_11213, mtgp32_params_fast_23209,
+                                                     mtgp32_params_fast_44497};
+        static mtgp64_params_fast_t *const t64[3] = {mtgp64_params_fast_23209, mtgp64_params_fast_44497,
+                                                     mtgp64_params_fast_110503};
+        memcpy(out, t < 3 ? t32[t][i].poly_sha1 : t64[t - 3][i].poly_sha1, 20);
+    } else {
+        memcpy(out, customs[(size_t)(slot - kNumSets)].sha1, 20);
+    }
+}
+
+DevParam dev_param_of(const mtgp32_params_fast_t &p) {
+    DevParam d{};
+    d.bits = 32;
+    d.mexp = p.mexp;
+    d.n = p.mexp / 32 + 1;
+    d.pos = p.pos;
+    d.sh1 = p.sh1;
+    d.sh2 = p.sh2;
+    d.mask_lo = p.mask;
+    for (int j = 0; j < 16; j++) {
+        d.tbl[j] = p.tbl[j];
+        d.tmp[j] = p.tmp_tbl[j];
+        d.flt[j] = p.flt_tmp_tbl[j];
+    }
+    return d;
+}
+
+DevParam dev_param_of(const mtgp64_params_fast_t &p) {
+    DevParam d{};
+    d.bits = 64;
+    d.mexp = p.mexp;
+    d.n = p.mexp / 64 + 1;
+    d.pos = p.pos;
+    d.sh1 = p.sh1;
+    d.sh2 = p.sh2;
+    d.mask_lo = (uint32_t)p.mask;
+    d.mask_hi = (uint32_t)(p.mask >> 32);
+    for (int j = 0; j < 16; j++) {
+        d.tbl[j] = (uint32_t)(p.tbl[j] >> 32);
+        d.tmp[j] = (uint32_t)(p.tmp_tbl[j] >> 32);
+        d.flt[j] = (uint32_t)(p.dbl_tmp_tbl[j] >> 32);
+    }
+    return d;
+}
+
+// The kernels keep only the high halves of MTGP64 tables; a custom set
+// with non-zero low halves cannot run on them.
+static void check64(const mtgp64_params_fast_t &p) {
+    for (int j = 0; j < 16; j++)
+        if ((uint32_t)p.tbl[j] || (uint32_t)p.tmp_tbl[j] || (uint32_t)p.dbl_tmp_tbl[j])
+            throw Error("mtgp64 parameter tables with non-zero low halves are not supported");
+}
+
+// ----------------------------------------------- characteristic polynomials
+
+namespace {
+
+// Host stream of `count` output LSBs from the window of `rec` (seed 1),
+// the sequence calc_characteristic feeds to MinPolySeq.
+std::vector<uint8_t> lsb_sequence(const DevParam &d, long count) {
+    std::vector<uint8_t> bits((size_t)count);
+    const int n = d.n, pos = d.pos;
+    if (d.bits == 32) {
+        std::vector<uint32_t> w((size_t)(n + count));
+        mtgp32_params_fast_t p{};
+        p.mexp = d.mexp;
+        for (int j = 0; j < 16; j++)
+            p.tbl[j] = d.tbl[j];
+        mtgp32_init_state(w.data(), &p, 1);
+        for (long k = 0; k < count; k++) {
+            uint32_t x = (w[(size_t)k] & d.mask_lo) ^ w[(size_t)k + 1];
+            x ^= x << d.sh1;
+            uint32_t y = x ^ (w[(size_t)(k + pos)] >> d.sh2);
+            y ^= d.tbl[y & 15];
+            w[(size_t)(n + k)] = y;
+            uint32_t t = w[(size_t)(k + pos - 1)];
+            t ^= t >> 16;
+            t ^= t >> 8;
+            bits[(size_t)k] = (uint8_t)((y ^ d.tmp[t & 15]) & 1);
+        }
+    } else {
+        std::vector<uint64_t> w((size_t)(n + count));
+        mtgp64_params_fast_t p{};
+        p.mexp = d.mexp;
+        for (int j = 0; j < 16; j++)
+            p.tbl[j] = (uint64_t)d.tbl[j] << 32;
+        mtgp64_init_state(w.data(), &p, 1);
+        const uint64_t mask = ((uint64_t)d.mask_hi << 32) | d.mask_lo;
+        for (long k = 0; k < count; k++) {
+            uint64_t x = (w[(size_t)k] & mask) ^ w[(size_t)k + 1];
+            uint32_t xh = (uint32_t)(x >> 32), xl = (uint32_t)x;
+            const uint64_t y = w[(size_t)(k + pos)];
+            xh ^= xh << d.sh1;
+            xl ^= xl << d.sh1;
+            const uint32_t nh = xl ^ ((uint32_t)(y >> 32) >> d.sh2);
+            const uint32_t nl = xh ^ ((uint32_t)y >> d.sh2);
+            w[(size_t)(n + k)] = ((uint64_t)(nh ^ d.tbl[nl & 15]) << 32) | nl;
+            // the tempering table's low halves are zero: the LSB is nl's
+            bits[(size_t)k] = (uint8_t)(nl & 1);
+        }
+    }
+    return bits;
+}
+
+std::mutex g_poly_mu;
+std::unordered_map<std::string, std::shared_ptr<gf2::Modulus>> g_mod;  // bits:mexp:sha1 -> P
+
+}  // namespace
+
+std::shared_ptr<gf2::Modulus> modulus_for(const DevParam &d, const unsigned char *sha1) {
+    char key[64];
+    snprintf(key, sizeof key, "%d:%d:", d.bits, d.mexp);
+    std::string k(key);
+    for (int i = 0; i < 20; i++) {
+        snprintf(key, sizeof key, "%02x", sha1[i]);
+        k += key;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_poly_mu);
+        auto it = g_mod.find(k);
+        if (it != g_mod.end())
+            return it->second;
+    }
+    gf2::Poly P = gf2::berlekamp_massey(lsb_sequence(d, 2L * d.mexp));
+    if (gf2::degree(P) != d.mexp)
+        throw Error("characteristic polynomial has degree " + std::to_string(gf2::degree(P)) + ", expected " +
+                    std::to_string(d.mexp));
+    const std::string s = gf2::coeff_string(P);
+    unsigned char h[20];
+    gf2::sha1(s.data(), s.size(), h);
+    if (memcmp(h, sha1, 20) != 0)
+        throw Error("characteristic polynomial does not match the parameter set's poly_sha1");
+    auto m = std::make_shared<gf2::Modulus>(P);
+    std::lock_guard<std::mutex> lk(g_poly_mu);
+    g_mod[k] = m;
+    return m;
+}
+
+std::shared_ptr<gf2::Modulus> modulus_for(Context &c, int slot) {
+    unsigned char sha[20];
+    c.sha1_of(slot, sha);
+    return modulus_for(c.param(slot), sha);
+}
+
+// ---------------------------------------------------------------- planning
+
+namespace {
+
+int elem_bytes(int bits) { return bits / 8; }
+
+// Parallel map over [0, n) on up to 16 host threads.
+template <class F>
+void parallel_for(int n, F f) {
+    int hw = (int)std::thread::hardware_concurrency();
+    int nt = std::max(1, std::min({n, hw > 0 ? hw : 1, 16}));
+    if (nt <= 1) {
+        for (int i = 0; i < n; i++)
+            f(i);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::vector<std::string> errs((size_t)nt);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            try {
+                for (int i; (i = next++) < n;)
+                    f(i);
+            } catch (const std::exception &e) {
+                errs[(size_t)t] = e.what();
+                next = n;
+            }
+        });
+    for (auto &x : th)
+        x.join();
+    for (auto &e : errs)
+        if (!e.empty())
+            throw Error(e);
+}
+
+}  // namespace
+
+uint64_t auto_segment(const Context &c, uint64_t total_numbers) {
+    // Enough segments to fill the GPU (about 8 per CU) but no shorter than
+    // 2^20 numbers, so that the start-state jump (about mexp/2 XORs of the
+    // window) stays a small fraction of a segment's work.
+    const uint64_t target = (uint64_t)c.cus * 8;
+    uint64_t s = (total_numbers + target - 1) / target;
+    s = std::max<uint64_t>(s, 1ull << 20);
+    return (s + 255) & ~255ull;
+}
+
+void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<mtgp_request_t> &reqs,
+                 uint64_t segment) {
+    const auto t0 = std::chrono::steady_clock::now();
+    ctx = &c;
+    // Per-generator request lists and the absolute stream offsets.
+    struct Piece {
+        int req;
+        uint64_t off, len, out_off;
+    };
+    const int G = (int)gens.size();
+    std::vector<std::vector<Piece>> pieces((size_t)G);
+    std::vector<uint64_t> total((size_t)G, 0);
+    std::vector<char> used((size_t)G, 0);
+    uint64_t all = 0;
+    for (const auto &r : reqs) {
+        if (r.gen < 0 || r.gen >= G)
+            throw Error("request for generator " + std::to_string(r.gen) + " outside the set");
+        if (r.kind < 0 || r.kind > 4)
+            throw Error("unknown output kind " + std::to_string(r.kind));
+        if (r.count && !r.out)
+            throw Error("request with a NULL output buffer");
+        if (r.count && ((uintptr_t)r.out % (uintptr_t)elem_bytes(gens[(size_t)r.gen].bits)))
+            throw Error("output buffer not aligned to its element size");
+        all += r.count;
+    }
+    const uint64_t S = segment ? segment : auto_segment(c, all);
+    for (int ri = 0; ri < (int)reqs.size(); ri++) {
+        const auto &r = reqs[(size_t)ri];
+        uint64_t &pos = total[(size_t)r.gen];
+        used[(size_t)r.gen] = 1;
+        pos += r.skip;
+        for (uint64_t done = 0; done < r.count;) {
+            const uint64_t len = std::min<uint64_t>(S, r.count - done);
+            pieces[(size_t)r.gen].push_back(Piece{ri, pos + done, len, done});
+            done += len;
+        }
+        pos += r.count;
+    }
+    // A pure skip at the end still has to move the state.
+    for (int g = 0; g < G; g++) {
+        if (!used[(size_t)g])
+            continue;
+        auto &pv = pieces[(size_t)g];
+        if (pv.empty() || pv.back().off + pv.back().len != total[(size_t)g])
+            pv.push_back(Piece{-1, total[(size_t)g], 0, 0});
+    }
+    // Jump polynomials x^off mod P for every piece with off > 0.
+    std::vector<std::vector<std::vector<uint32_t>>> polys((size_t)G);
+    parallel_for(G, [&](int g) {
+        const auto &pv = pieces[(size_t)g];
+        auto &out = polys[(size_t)g];
+        out.resize(pv.size());
+        bool any = false;
+        for (const auto &p : pv)
+            any |= p.off > 0;
+        if (!any)
+            return;
+        const GenRef &gr = gens[(size_t)g];
+        auto mod = modulus_for(c, gr.slot);
+        const int pw = (gr.mexp + 31) / 32;
+        std::map<uint64_t, gf2::Poly> step_cache;
+        gf2::Poly cur;
+        uint64_t cur_off = 0;
+        for (size_t k = 0; k < pv.size(); k++) {
+            const uint64_t off = pv[k].off;
+            if (off == 0)
+                continue;
+            if (cur_off == 0) {
+                cur = mod->xpow(off);
+            } else if (off != cur_off) {
+                const uint64_t delta = off - cur_off;
+                auto it = step_cache.find(delta);
+                if (it == step_cache.end())
+                    it = step_cache.emplace(delta, mod->xpow(delta)).first;
+                cur = mod->mulmod(cur, it->second);
+            }
+            cur_off = off;
+            out[k] = gf2::to_words32(cur, pw);
+        }
+    });
+    // Items, grouped by (word size, kernel class), longest first.
+    std::vector<Item> list;
+    std::vector<int> group;
+    std::vector<uint32_t> poly_host;
+    for (int g = 0; g < G; g++) {
+        const GenRef &gr = gens[(size_t)g];
+        for (size_t k = 0; k < pieces[(size_t)g].size(); k++) {
+            const Piece &p = pieces[(size_t)g][k];
+            Item it{};
+            it.count = p.len;
+            it.kind = p.req >= 0 ? reqs[(size_t)p.req].kind : 0;
+            it.out = p.req >= 0 ? (uint64_t)(uintptr_t)reqs[(size_t)p.req].out + p.out_off * (uint64_t)elem_bytes(gr.bits)
+                                : 0;
+            it.state_in = (uint64_t)(uintptr_t)gr.state_in;
+            it.state_out = (p.off + p.len == total[(size_t)g]) ? (uint64_t)(uintptr_t)gr.state_out : 0;
+            it.param = gr.slot;
+            if (p.off > 0) {
+                const auto &pw = polys[(size_t)g][k];
+                it.poly = poly_host.size();  // word offset, fixed up below
+                it.poly_words = (int32_t)pw.size();
+                poly_host.insert(poly_host.end(), pw.begin(), pw.end());
+            } else {
+                it.poly = UINT64_MAX;
+            }
+            list.push_back(it);
+            group.push_back((gr.bits == 64 ? 3 : 0) + kernel_class(gr.bits, gr.n));
+            if (it.state_out)
+                commits.push_back(g);
+        }
+    }
+    std::vector<int> order(list.size());
+    for (size_t i = 0; i < order.size(); i++)
+        order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (group[(size_t)a] != group[(size_t)b])
+            return group[(size_t)a] < group[(size_t)b];
+        // cost ~ numbers + a jump's worth when there is one
+        const uint64_t ca = list[(size_t)a].count + (list[(size_t)a].poly != UINT64_MAX ? 200000u : 0u);
+        const uint64_t cb = list[(size_t)b].count + (list[(size_t)b].poly != UINT64_MAX ? 200000u : 0u);
+        return ca > cb;
+    });
+    // Device copies.
+    free_device();
+    if (!poly_host.empty()) {
+        hip_check(hipMalloc((void **)&d_polys, poly_host.size() * 4), "hipMalloc polys");
+        hip_check(hipMemcpy(d_polys, poly_host.data(), poly_host.size() * 4, hipMemcpyHostToDevice), "hipMemcpy polys");
+    }
+    items.clear();
+    for (int i : order) {
+        Item it = list[(size_t)i];
+        it.poly = it.poly == UINT64_MAX ? 0 : (uint64_t)(uintptr_t)(d_polys + it.poly);
+        items.push_back(it);
+    }
+    launches.clear();
+    for (size_t i = 0; i < order.size();) {
+        const int gi = group[(size_t)order[i]];
+        size_t j = i;
+        while (j < order.size() && group[(size_t)order[j]] == gi)
+            j++;
+        Launch L;
+        L.bits = gi >= 3 ? 64 : 32;
+        L.cls = gi % 3;
+        L.first = (int)i;
+        L.count = (int)(j - i);
+        const int cap = c.cus * c.blocks_per_cu[L.bits == 64][L.cls];
+        L.grid = std::min(L.count, cap);
+        launches.push_back(L);
+        i = j;
+    }
+    if (!items.empty()) {
+        hip_check(hipMalloc((void **)&d_items, items.size() * sizeof(Item)), "hipMalloc items");
+        hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice),
+                  "hipMemcpy items");
+        hip_check(hipMalloc((void **)&d_queue, launches.size() * 2 * sizeof(uint32_t)), "hipMalloc queue");
+        hip_check(hipMemset(d_queue, 0, launches.size() * 2 * sizeof(uint32_t)), "hipMemset queue");
+    }
+    segment_used = S;
+    plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void Plan::launch(hipStream_t s) {
+    for (size_t k = 0; k < launches.size(); k++) {
+        const Launch &L = launches[k];
+        hip_check(launch_items(L.bits, L.cls, d_items + L.first, L.count, ctx->d_params, d_queue + 2 * k, L.grid, s),
+                  "kernel launch");
+    }
+}
+
+void Plan::free_device() {
+    if (d_items)
+        (void)hipFree(d_items);
+    if (d_polys)
+        (void)hipFree(d_polys);
+    if (d_queue)
+        (void)hipFree(d_queue);
+    d_items = nullptr;
+    d_polys = nullptr;
+    d_queue = nullptr;
+}
+
+Plan::~Plan() { free_device(); }
+
+}  // namespace mtgp
+
+using namespace mtgp;
+
+// ------------------------------------------------------------------ C API
+
+#define API_BEGIN try {
+#define API_END(fail)                   \
+    }                                   \
+    catch (const std::exception &e) {   \
+        set_error(e.what());            \
+        return fail;                    \
+    }
+
+struct mtgp_genset {
+    Context *ctx;
+    std::vector<int> bits, mexp, slot, n, cur;
+    std::vector<size_t> off;  // byte offset of slot pair in d_states
+    char *d_states = nullptr;
+    ~mtgp_genset() {
+        if (d_states)
+            (void)hipFree(d_states);
+    }
+    void *state(int g, int which) {
+        return d_states + off[(size_t)g] + (size_t)which * (size_t)n[(size_t)g] * (size_t)(bits[(size_t)g] / 8);
+    }
+    std::vector<GenRef> refs() {
+        std::vector<GenRef> r(bits.size());
+        for (size_t g = 0; g < bits.size(); g++) {
+            r[g].bits = bits[g];
+            r[g].mexp = mexp[g];
+            r[g].n = n[g];
+            r[g].slot = slot[g];
+            r[g].state_in = state((int)g, cur[g]);
+            r[g].state_out = state((int)g, cur[g] ^ 1);
+        }
+        return r;
+    }
+};
+
+struct mtgp_plan {
+    mtgp_genset *set;
+    Plan plan;
+};
+
+// Applies a jump polynomial (hex) to device window `in`, result in `out`.
+static void device_jump(Context &c, int slot, int bits, int mexp, const void *in, void *out, const char *hex) {
+    gf2::Poly P;
+    if (!gf2::from_hex(hex, P))
+        throw Error("jump string contains a non-hex character");
+    const int pw = std::max<int>(1, (int)((strlen(hex) * 4 + 31) / 32));
+    std::vector<uint32_t> words = gf2::to_words32(P, pw);
+    uint32_t *d_poly = nullptr;
+    Item *d_item = nullptr;
+    uint32_t *d_q = nullptr;
+    try {
+        hip_check(hipMalloc((void **)&d_poly, words.size() * 4), "hipMalloc");
+        hip_check(hipMemcpy(d_poly, words.data(), words.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        Item it{};
+        it.state_in = (uint64_t)(uintptr_t)in;
+        it.state_out = (uint64_t)(uintptr_t)out;
+        it.poly = (uint64_t)(uintptr_t)d_poly;
+        it.poly_words = pw;
+        it.param = slot;
+        hip_check(hipMalloc((void **)&d_item, sizeof(Item)), "hipMalloc");
+        hip_check(hipMemcpy(d_item, &it, sizeof(Item), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
+        hip_check(hipMemset(d_q, 0, 8), "hipMemset");
+        hip_check(launch_items(bits, kernel_class(bits, state_words(bits, mexp)), d_item, 1, c.d_params, d_q, 1, 0),
+                  "kernel launch");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    } catch (...) {
+        (void)hipFree(d_poly);
+        (void)hipFree(d_item);
+        (void)hipFree(d_q);
+        throw;
+    }
+    (void)hipFree(d_poly);
+    (void)hipFree(d_item);
+    (void)hipFree(d_q);
+}
+
+template <class Gen, class Word>
+static int fill_impl(Gen *g, int kind, void *out, size_t n, int bits) {
+    Context &c = context();
+    if (kind < 0 || kind > 4)
+        throw Error("unknown output kind");
+    if (!g || !g->status)
+        throw Error("generator is not initialised");
+    const int N = g->status->size;
+    DevParam rec = dev_param_of(g->params);
+    const int slot = c.slot_of(rec, g->params.poly_sha1);
+    std::vector<Word> w((size_t)N);
+    for (int j = 0; j < N; j++)
+        w[(size_t)j] = g->status->array[(g->status->idx - N + 1 + j + g->status->large_size) & g->status->large_mask];
+    const size_t eb = (size_t)bits / 8;
+    bool dev_out = false;
+    if (out) {
+        hipPointerAttribute_t attr;
+        if (hipPointerGetAttributes(&attr, out) == hipSuccess &&
+            (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged))
+            dev_out = true;
+        (void)hipGetLastError();
+    }
+    if (n && !out)
+        throw Error("NULL output buffer");
+    char *d_states = nullptr;
+    void *d_out = nullptr;
+    auto cleanup = [&] {
+        (void)hipFree(d_states);
+        if (!dev_out)
+            (void)hipFree(d_out);
+    };
+    try {
+        hip_check(hipMalloc((void **)&d_states, 2 * (size_t)N * sizeof(Word)), "hipMalloc");
+        hip_check(hipMemcpy(d_states, w.data(), (size_t)N * sizeof(Word), hipMemcpyHostToDevice), "hipMemcpy");
+        if (dev_out)
+            d_out = out;
+        else if (n)
+            hip_check(hipMalloc(&d_out, n * eb), "hipMalloc");
+        GenRef r;
+        r.bits = bits;
+        r.mexp = g->params.mexp;
+        r.n = N;
+        r.slot = slot;
+        r.state_in = d_states;
+        r.state_out = d_states + (size_t)N * sizeof(Word);
+        mtgp_request_t q{0, kind, (uint64_t)n, 0, d_out};
+        Plan plan;
+        plan.build(c, {r}, {q}, 0);
+        plan.launch(0);
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        hip_check(hipMemcpy(w.data(), r.state_out, (size_t)N * sizeof(Word), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (n && !dev_out)
+            hip_check(hipMemcpy(out, d_out, n * eb, hipMemcpyDeviceToHost), "hipMemcpy");
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+    memcpy(g->status->array, w.data(), (size_t)N * sizeof(Word));
+    g->status->idx = N - 1;
+    return 0;
+}
+
+
+template <class Gen, class Word>
+static void fast_jump_impl(Gen *g, const char *hex, int bits) {
+    Context &c = context();
+    const int N = g->status->size;
+    DevParam rec = dev_param_of(g->params);
+    const int slot = c.slot_of(rec, g->params.poly_sha1);
+    std::vector<Word> w((size_t)N);
+    for (int j = 0; j < N; j++)
+        w[(size_t)j] = g->status->array[(g->status->idx - N + 1 + j + g->status->large_size) & g->status->large_mask];
+    char *d = nullptr;
+    hip_check(hipMalloc((void **)&d, 2 * (size_t)N * sizeof(Word)), "hipMalloc");
+    try {
+        hip_check(hipMemcpy(d, w.data(), (size_t)N * sizeof(Word), hipMemcpyHostToDevice), "hipMemcpy");
+        device_jump(c, slot, bits, g->params.mexp, d, d + (size_t)N * sizeof(Word), hex);
+        hip_check(hipMemcpy(w.data(), d + (size_t)N * sizeof(Word), (size_t)N * sizeof(Word), hipMemcpyDeviceToHost),
+                  "hipMemcpy");
+    } catch (...) {
+        (void)hipFree(d);
+        throw;
+    }
+    (void)hipFree(d);
+    memcpy(g->status->array, w.data(), (size_t)N * sizeof(Word));
+    g->status->idx = N - 1;
+}
+
+
+static int put_string(const std::string &s, char *buf, size_t cap) {
+    if (!buf || s.size() + 1 > cap) {
+        set_error("buffer too small");
+        return -1;
+    }
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return (int)s.size();
+}
+
+// Characteristic and jump polynomials are host arithmetic: no GPU needed.
+template <class Params>
+static int charpoly_impl(const Params &p, char *buf, size_t cap) {
+    auto m = modulus_for(dev_param_of(p), p.poly_sha1);
+    return put_string(gf2::to_hex(m->poly()), buf, cap);
+}
+
+template <class Params>
+static int jump_impl(const Params &p, const uint64_t *steps, int nwords, char *buf, size_t cap) {
+    if (nwords <= 0)
+        throw Error("empty step count");
+    auto m = modulus_for(dev_param_of(p), p.poly_sha1);
+    std::vector<uint64_t> e(steps, steps + nwords);
+    return put_string(gf2::to_hex(m->xpow(e)), buf, cap);
+}
+
+extern "C" {
+
+const char *mtgp_last_error(void) { return last_error(); }
+
+int mtgp_set_device(int device) {
+    API_BEGIN
+    int count = 0;
+    hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count)
+        throw Error("no such device");
+    select_device(device);
+    context();
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_device_info(int *cus, size_t *hbm) {
+    API_BEGIN
+    Context &c = context();
+    if (cus)
+        *cus = c.cus;
+    if (hbm)
+        *hbm = c.hbm_bytes;
+    return 0;
+    API_END(-1)
+}
+
+void *mtgp_dev_malloc(size_t bytes) {
+    API_BEGIN
+    context();
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc");
+    return p;
+    API_END(nullptr)
+}
+
+int mtgp_dev_free(void *p) {
+    API_BEGIN
+    hip_check(hipFree(p), "hipFree");
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    API_BEGIN
+    hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    API_BEGIN
+    hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_synchronize(void) {
+    API_BEGIN
+    context();
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return 0;
+    API_END(-1)
+}
+
+void *mtgp_stream_create(void) {
+    API_BEGIN
+    context();
+    hipStream_t s;
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    return (void *)s;
+    API_END(nullptr)
+}
+
+int mtgp_stream_destroy(void *s) {
+    API_BEGIN
+    hip_check(hipStreamDestroy((hipStream_t)s), "hipStreamDestroy");
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_stream_synchronize(void *s) {
+    API_BEGIN
+    hip_check(hipStreamSynchronize((hipStream_t)s), "hipStreamSynchronize");
+    return 0;
+    API_END(-1)
+}
+
+void *mtgp_event_create(void) {
+    API_BEGIN
+    context();
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return (void *)e;
+    API_END(nullptr)
+}
+
+int mtgp_event_destroy(void *e) {
+    API_BEGIN
+    hip_check(hipEventDestroy((hipEvent_t)e), "hipEventDestroy");
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_event_record(void *e, void *s) {
+    API_BEGIN
+    hip_check(hipEventRecord((hipEvent_t)e, (hipStream_t)s), "hipEventRecord");
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_event_elapsed_ms(void *a, void *b, float *ms) {
+    API_BEGIN
+    hip_check(hipEventSynchronize((hipEvent_t)b), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b), "hipEventElapsedTime");
+    return 0;
+    API_END(-1)
+}
+
+// ------------------------------------------------------------ gensets
+
+mtgp_genset_t *mtgp_genset_create(int count, const int *bits, const int *mexp, const int *index) {
+    API_BEGIN
+    if (count <= 0)
+        throw Error("generator count must be positive");
+    Context &c = context();
+    std::unique_ptr<mtgp_genset> s(new mtgp_genset());
+    s->ctx = &c;
+    size_t bytes = 0;
+    for (int g = 0; g < count; g++) {
+        const int slot = global_set_id(bits[g], mexp[g], index[g]);
+        if (slot < 0)
+            throw Error("no parameter set mtgp" + std::to_string(bits[g]) + "-" + std::to_string(mexp[g]) + " #" +
+                        std::to_string(index[g]));
+        s->bits.push_back(bits[g]);
+        s->mexp.push_back(mexp[g]);
+        s->slot.push_back(slot);
+        s->n.push_back(state_words(bits[g], mexp[g]));
+        s->cur.push_back(0);
+        s->off.push_back(bytes);
+        bytes += 2 * (size_t)s->n.back() * (size_t)(bits[g] / 8);
+        bytes = (bytes + 255) & ~(size_t)255;
+    }
+    hip_check(hipMalloc((void **)&s->d_states, bytes), "hipMalloc states");
+    hip_check(hipMemset(s->d_states, 0, bytes), "hipMemset states");
+    return s.release();
+    API_END(nullptr)
+}
+
+void mtgp_genset_destroy(mtgp_genset_t *s) { delete s; }
+
+int mtgp_genset_count(const mtgp_genset_t *s) { return (int)s->bits.size(); }
+
+int mtgp_genset_state_words(const mtgp_genset_t *s, int g) {
+    if (g < 0 || g >= (int)s->bits.size()) {
+        set_error("generator index out of range");
+        return -1;
+    }
+    return s->n[(size_t)g];
+}
+
+static void upload_window(mtgp_genset *s, int g, const void *w) {
+    s->cur[(size_t)g] = 0;
+    hip_check(hipMemcpy(s->state(g, 0), w, (size_t)s->n[(size_t)g] * (size_t)(s->bits[(size_t)g] / 8),
+                        hipMemcpyHostToDevice),
+              "hipMemcpy state");
+}
+
+int mtgp_genset_init_seed(mtgp_genset_t *s, const uint64_t *seeds) {
+    API_BEGIN
+    for (int g = 0; g < (int)s->bits.size(); g++) {
+        const int t = s->slot[(size_t)g] / kSetsPerTable, i = s->slot[(size_t)g] % kSetsPerTable;
+        if (s->bits[(size_t)g] == 32) {
+            static mtgp32_params_fast_t *const t32[3] = {mtgp32_params_fast_11213, mtgp32_params_fast_23209,
+                                                         mtgp32_params_fast_44497};
+            std::vector<uint32_t> w((size_t)s->n[(size_t)g]);
+            mtgp32_init_state(w.data(), &t32[t][i], (uint32_t)seeds[g]);
+            upload_window(s, g, w.data());
+        } else {
+            static mtgp64_params_fast_t *const t64[3] = {mtgp64_params_fast_23209, mtgp64_params_fast_44497,
+                                                         mtgp64_params_fast_110503};
+            std::vector<uint64_t> w((size_t)s->n[(size_t)g]);
+            mtgp64_init_state(w.data(), &t64[t - 3][i], seeds[g]);
+            upload_window(s, g, w.data());
+        }
+    }
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_genset_init_array(mtgp_genset_t *s, const uint64_t *keys, const int *offsets, const int *lengths) {
+    API_BEGIN
+    for (int g = 0; g < (int)s->bits.size(); g++) {
+        const int t = s->slot[(size_t)g] / kSetsPerTable, i = s->slot[(size_t)g] % kSetsPerTable;
+        const uint64_t *k = keys + offsets[g];
+        if (s->bits[(size_t)g] == 32) {
+            static mtgp32_params_fast_t *const t32[3] = {mtgp32_params_fast_11213, mtgp32_params_fast_23209,
+                                                         mtgp32_params_fast_44497};
+            std::vector<uint32_t> key((size_t)lengths[g]);
+            for (int j = 0; j < lengths[g]; j++)
+                key[(size_t)j] = (uint32_t)k[j];
+            mtgp32_fast_t tmp;
+            if (mtgp32_init_by_array(&tmp, &t32[t][i], key.data(), lengths[g]))
+                throw Error("out of host memory");
+            upload_window(s, g, tmp.status->array);
+            mtgp32_free(&tmp);
+        } else {
+            static mtgp64_params_fast_t *const t64[3] = {mtgp64_params_fast_23209, mtgp64_params_fast_44497,
+                                                         mtgp64_params_fast_110503};
+            std::vector<uint64_t> key(k, k + lengths[g]);
+            mtgp64_fast_t tmp;
+            if (mtgp64_init_by_array(&tmp, &t64[t - 3][i], key.data(), lengths[g]))
+                throw Error("out of host memory");
+            upload_window(s, g, tmp.status->array);
+            mtgp64_free(&tmp);
+        }
+    }
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_genset_set_state(mtgp_genset_t *s, int g, const void *w) {
+    API_BEGIN
+    if (g < 0 || g >= (int)s->bits.size())
+        throw Error("generator index out of range");
+    upload_window(s, g, w);
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_genset_get_state(mtgp_genset_t *s, int g, void *w) {
+    API_BEGIN
+    if (g < 0 || g >= (int)s->bits.size())
+        throw Error("generator index out of range");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    hip_check(hipMemcpy(w, s->state(g, s->cur[(size_t)g]), (size_t)s->n[(size_t)g] * (size_t)(s->bits[(size_t)g] / 8),
+                        hipMemcpyDeviceToHost),
+              "hipMemcpy state");
+    return 0;
+    API_END(-1)
+}
+
+mtgp_plan_t *mtgp_plan_create(mtgp_genset_t *s, const mtgp_request_t *reqs, int nreq, uint64_t segment) {
+    API_BEGIN
+    std::unique_ptr<mtgp_plan> p(new mtgp_plan());
+    p->set = s;
+    p->plan.build(*s->ctx, s->refs(), std::vector<mtgp_request_t>(reqs, reqs + nreq), segment);
+    return p.release();
+    API_END(nullptr)
+}
+
+int mtgp_plan_launch(mtgp_plan_t *p, void *stream) {
+    API_BEGIN
+    p->plan.launch((hipStream_t)stream);
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_plan_commit(mtgp_plan_t *p) {
+    API_BEGIN
+    for (int g : p->plan.commits)
+        p->set->cur[(size_t)g] ^= 1;
+    p->plan.commits.clear();
+    return 0;
+    API_END(-1)
+}
+
+void mtgp_plan_destroy(mtgp_plan_t *p) { delete p; }
+
+int mtgp_plan_stats(const mtgp_plan_t *p, int *items, int *launches, double *plan_ms) {
+    if (items)
+        *items = (int)p->plan.items.size();
+    if (launches)
+        *launches = (int)p->plan.launches.size();
+    if (plan_ms)
+        *plan_ms = p->plan.plan_ms;
+    return 0;
+}
+
+int mtgp_genset_generate(mtgp_genset_t *s, const mtgp_request_t *reqs, int nreq, void *stream) {
+    API_BEGIN
+    mtgp_plan p;
+    p.set = s;
+    p.plan.build(*s->ctx, s->refs(), std::vector<mtgp_request_t>(reqs, reqs + nreq), 0);
+    p.plan.launch((hipStream_t)stream);
+    hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    for (int g : p.plan.commits)
+        s->cur[(size_t)g] ^= 1;
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_genset_jump(mtgp_genset_t *s, int g, const char *hex) {
+    API_BEGIN
+    if (g < 0 || g >= (int)s->bits.size())
+        throw Error("generator index out of range");
+    const int cur = s->cur[(size_t)g];
+    device_jump(*s->ctx, s->slot[(size_t)g], s->bits[(size_t)g], s->mexp[(size_t)g], s->state(g, cur),
+                s->state(g, cur ^ 1), hex);
+    s->cur[(size_t)g] ^= 1;
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) {
+    API_BEGIN
+    const int G = (int)s->bits.size();
+    for (int g = 1; g < G; g++)
+        if (s->slot[(size_t)g] != s->slot[0])
+            throw Error("substreams need one parameter set for all generators");
+    gf2::Poly J;
+    if (!gf2::from_hex(hex, J))
+        throw Error("jump string contains a non-hex character");
+    Context &c = *s->ctx;
+    auto mod = modulus_for(c, s->slot[0]);
+    J = mod->reduce(J);
+    // c_i = J^i mod P, chained in blocks so that host threads share the work.
+    const int pw = (s->mexp[0] + 31) / 32;
+    std::vector<std::vector<uint32_t>> polys((size_t)G);
+    const int blk = 64;
+    const int nblk = (G + blk - 1) / blk;
+    std::vector<gf2::Poly> starts((size_t)nblk);
+    {
+        gf2::Poly Jb(mod->words(), 0);
+        Jb[0] = 1;
+        for (int k = 0; k < blk; k++)
+            Jb = mod->mulmod(Jb, J);  // J^blk
+        gf2::Poly cur(mod->words(), 0);
+        cur[0] = 1;
+        for (int b = 0; b < nblk; b++) {
+            starts[(size_t)b] = cur;
+            cur = mod->mulmod(cur, Jb);
+        }
+    }
+    parallel_for(nblk, [&](int b) {
+        gf2::Poly cur = starts[(size_t)b];
+        for (int g = b * blk; g < std::min(G, (b + 1) * blk); g++) {
+            polys[(size_t)g] = gf2::to_words32(cur, pw);
+            cur = mod->mulmod(cur, J);
+        }
+    });
+    // Base window to the device, then one jump item per generator.
+    const size_t wbytes = (size_t)s->n[0] * (size_t)(s->bits[0] / 8);
+    void *d_base = nullptr;
+    uint32_t *d_polys = nullptr, *d_q = nullptr;
+    Item *d_items = nullptr;
+    auto cleanup = [&] {
+        (void)hipFree(d_base);
+        (void)hipFree(d_polys);
+        (void)hipFree(d_q);
+        (void)hipFree(d_items);
+    };
+    try {
+        hip_check(hipMalloc(&d_base, wbytes), "hipMalloc");
+        hip_check(hipMemcpy(d_base, base, wbytes, hipMemcpyHostToDevice), "hipMemcpy");
+        std::vector<uint32_t> flat((size_t)G * (size_t)pw);
+        for (int g = 0; g < G; g++)
+            std::copy(polys[(size_t)g].begin(), polys[(size_t)g].end(), flat.begin() + (size_t)g * (size_t)pw);
+        hip_check(hipMalloc((void **)&d_polys, flat.size() * 4), "hipMalloc");
+        hip_check(hipMemcpy(d_polys, flat.data(), flat.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        std::vector<Item> items((size_t)G);
+        for (int g = 0; g < G; g++) {
+            Item &it = items[(size_t)g];
+            it = Item{};
+            it.state_in = (uint64_t)(uintptr_t)d_base;
+            it.state_out = (uint64_t)(uintptr_t)s->state(g, 0);
+            if (g > 0) {
+                it.poly = (uint64_t)(uintptr_t)(d_polys + (size_t)g * (size_t)pw);
+                it.poly_words = pw;
+            }
+            it.param = s->slot[0];
+        }
+        hip_check(hipMalloc((void **)&d_items, items.size() * sizeof(Item)), "hipMalloc");
+        hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
+        hip_check(hipMemset(d_q, 0, 8), "hipMemset");
+        const int bits = s->bits[0], cls = kernel_class(bits, s->n[0]);
+        const int grid = std::min(G, c.cus * c.blocks_per_cu[bits == 64][cls]);
+        hip_check(launch_items(bits, cls, d_items, G, c.d_params, d_q, grid, 0), "kernel launch");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+    for (int g = 0; g < G; g++)
+        s->cur[(size_t)g] = 0;
+    return 0;
+    API_END(-1)
+}
+
+// ------------------------------------------- host-struct bulk generation
+
+int mtgp32_fill(mtgp32_fast_t *g, int kind, void *out, size_t n) {
+    API_BEGIN
+    return fill_impl<mtgp32_fast_t, uint32_t>(g, kind, out, n, 32);
+    API_END(-1)
+}
+
+int mtgp64_fill(mtgp64_fast_t *g, int kind, void *out, size_t n) {
+    API_BEGIN
+    if (g)
+        check64(g->params);
+    return fill_impl<mtgp64_fast_t, uint64_t>(g, kind, out, n, 64);
+    API_END(-1)
+}
+
+// ---------------------------------------------- jump drop-ins and polys
+
+// Errors cannot be returned through the reference signature: the state is
+// then left unchanged and mtgp_last_error() is non-empty ("" on success).
+void mtgp32_fast_jump(mtgp32_fast_t *g, const char *hex) {
+    try {
+        fast_jump_impl<mtgp32_fast_t, uint32_t>(g, hex, 32);
+        set_error("");
+    } catch (const std::exception &e) {
+        set_error(e.what());
+    }
+}
+
+void mtgp64_fast_jump(mtgp64_fast_t *g, const char *hex) {
+    try {
+        check64(g->params);
+        fast_jump_impl<mtgp64_fast_t, uint64_t>(g, hex, 64);
+        set_error("");
+    } catch (const std::exception &e) {
+        set_error(e.what());
+    }
+}
+
+int mtgp32_charpoly_hex(const mtgp32_params_fast_t *p, char *buf, size_t cap) {
+    API_BEGIN
+    return charpoly_impl(*p, buf, cap);
+    API_END(-1)
+}
+
+int mtgp64_charpoly_hex(const mtgp64_params_fast_t *p, char *buf, size_t cap) {
+    API_BEGIN
+    check64(*p);
+    return charpoly_impl(*p, buf, cap);
+    API_END(-1)
+}
+
+int mtgp32_jump_hex(const mtgp32_params_fast_t *p, const uint64_t *steps, int nwords, char *buf, size_t cap) {
+    API_BEGIN
+    return jump_impl(*p, steps, nwords, buf, cap);
+    API_END(-1)
+}
+
+int mtgp64_jump_hex(const mtgp64_params_fast_t *p, const uint64_t *steps, int nwords, char *buf, size_t cap) {
+    API_BEGIN
+    check64(*p);
+    return jump_impl(*p, steps, nwords, buf, cap);
+    API_END(-1)
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the oracle.
+
+Every comparison is bit-exact: integers and the raw bits of every float or
+double output.  Sizes are chosen so the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SETS = [(32, 11213), (32, 23209), (32, 44497), (64, 23209), (64, 44497), (64, 110503)]
+
+
+def bits_of(a):
+    return a.view(np.uint32 if a.dtype.itemsize == 4 else np.uint64)
+
+
+def assert_same(got, want, what):
+    g, w = bits_of(np.asarray(got)), bits_of(np.asarray(want))
+    assert g.shape == w.shape, what
+    bad = np.nonzero(g != w)[0]
+    assert bad.size == 0, "%s: %d mismatches, first at %d: %x != %x" % (
+        what, bad.size, bad[0], int(g[bad[0]]), int(w[bad[0]]))
+
+
+@pytest.mark.parametrize("bits,mexp", SETS)
+@pytest.mark.parametrize("index", [0, 127])
+def test_fill_sequence_all_kinds(mtgp, oracle, bits, mexp, index):
+    """Short fills of every kind, continuing one stream, with lengths that
+    hit the head (unaligned), full-step and tail paths."""
+    gen = (mtgp.Generator32 if bits == 32 else mtgp.Generator64)(mexp, index, seed=4321 + index)
+    p = oracle.params(bits, mexp, index)
+    w = oracle.init_state(p, 4321 + index)
+    assert_same(gen.window(), w, "init")
+    for kind, n in [(0, 1), (1, 3), (2, 255), (3, 256), (4, 257), (0, 1000), (2, 5000), (1, 0), (0, 777)]:
+        got = gen.fill(kind, n)
+        want, w = oracle.generate(p, w, kind, n)
+        assert_same(got, want, "kind %d n %d" % (kind, n))
+        assert_same(gen.window(), w, "window after kind %d n %d" % (kind, n))
+
+
+def test_config1_single_generator(mtgp, oracle):
+    """MTGP32-11213 set 0, mtgp32_init(1234): 2^24 uint32 then 2^24
+    close1_open2 continuing the stream, bit-exact, with the final state."""
+    n = 1 << 24
+    gen = mtgp.Generator32(11213, 0, seed=1234)
+    p = oracle.params(32, 11213, 0)
+    w = oracle.init_state(p, 1234)
+    got = gen.fill(0, n)
+    want, w = oracle.generate(p, w, 0, n)
+    assert_same(got, want, "uint32")
+    got = gen.fill(1, n)
+    want, w = oracle.generate(p, w, 1, n)
+    assert_same(got, want, "close1_open2")
+    assert_same(gen.window(), w, "final window")
+
+
+@pytest.mark.parametrize("offset", [0, 4, 8, 12])
+def test_unaligned_device_output(mtgp, oracle, offset):
+    n = 10000
+    buf = mtgp.DeviceBuffer(n * 4 + 64)
+    gen = mtgp.Generator32(11213, 5, seed=99)
+    gen.fill(0, n, out=_View(buf, offset))
+    got = buf.to_host(np.uint32, n, offset)
+    p = oracle.params(32, 11213, 5)
+    want, _ = oracle.generate(p, oracle.init_state(p, 99), 0, n)
+    assert_same(got, want, "offset %d" % offset)
+
+
+@pytest.mark.parametrize("offset", [0, 8])
+def test_unaligned_device_output_64(mtgp, oracle, offset):
+    n = 3001
+    buf = mtgp.DeviceBuffer(n * 8 + 64)
+    gen = mtgp.Generator64(23209, 7, seed=5)
+    gen.fill(2, n, out=_View(buf, offset))
+    got = buf.to_host(np.float64, n, offset)
+    p = oracle.params(64, 23209, 7)
+    want, _ = oracle.generate(p, oracle.init_state(p, 5), 2, n)
+    assert_same(got, want, "offset %d" % offset)
+
+
+class _View:
+    """A DeviceBuffer seen from a byte offset (for fill(out=...))."""
+
+    def __init__(self, buf, offset):
+        self.ptr = buf.ptr + offset
+        self.buf = buf
+
+
+def test_fast_jump_matches_oracle(mtgp, oracle):
+    hexs = mtgp.jump_hex(32, 11213, 0, 3**162)
+    gen = mtgp.Generator32(11213, 0, seed=777)
+    p = oracle.params(32, 11213, 0)
+    w = oracle.init_state(p, 777)
+    gen.fast_jump(hexs)
+    w = oracle.jump(p, w, hexs)
+    assert_same(gen.window(), w, "jumped window")
+    got = gen.fill(0, 5000)
+    want, _ = oracle.generate(p, w, 0, 5000)
+    assert_same(got, want, "after jump")
+
+
+@pytest.mark.parametrize("bits,mexp", SETS)
+def test_jump_equals_stepping(mtgp, oracle, bits, mexp):
+    """test-jump32.cpp:218-264 pattern: jumping by x^s mod P equals
+    stepping s numbers, for s around the window and step sizes."""
+    n = mexp // bits + 1
+    for s in (1, 2, n + 1, 32 * n - 1, 3003, 200004):
+        gen = (mtgp.Generator32 if bits == 32 else mtgp.Generator64)(mexp, 1, seed=0)
+        gen.fill(0, 3)
+        ref = gen.window()
+        p = oracle.params(bits, mexp, 1)
+        _, stepped = oracle.generate(p, ref, 0, s)
+        gen.fast_jump(mtgp.jump_hex(bits, mexp, 1, s))
+        a = gen.fill(0, 100)
+        b, _ = oracle.generate(p, stepped, 0, 100)
+        assert_same(a, b, "s=%d" % s)
+
+
+def test_genset_segmented_requests(mtgp, oracle):
+    """Several generators, requests with skips, split into short segments so
+    that most segments start from a device jump."""
+    specs = [(32, 11213, 3), (32, 23209, 9), (64, 23209, 4), (32, 11213, 100), (64, 44497, 2)]
+    gs = mtgp.GenSet(specs)
+    seeds = [11, 22, 33, 44, 55]
+    gs.init_seed(seeds)
+    plan_reqs, expect, bufs = [], [], []
+    windows = {}
+    for g, (bits, mexp, idx) in enumerate(specs):
+        p = oracle.params(bits, mexp, idx)
+        windows[g] = (p, oracle.init_state(p, seeds[g]))
+    layout = [(0, 0, 70000, 0), (0, 2, 30001, 1234), (1, 1, 50000, 7), (2, 0, 40000, 0),
+              (3, 4, 123457, 99999), (2, 3, 1, 3), (4, 2, 20000, 0), (4, 0, 5, 5)]
+    for g, kind, count, skip in layout:
+        bits = specs[g][0]
+        dt = mtgp.out_dtype(bits, kind)
+        buf = mtgp.DeviceBuffer(count * np.dtype(dt).itemsize)
+        bufs.append((buf, dt, count))
+        plan_reqs.append((g, kind, count, skip, buf))
+        p, w = windows[g]
+        if skip:
+            _, w = oracle.generate(p, w, 0, skip)
+        want, w = oracle.generate(p, w, kind, count)
+        windows[g] = (p, w)
+        expect.append(want)
+    plan = gs.plan(plan_reqs, segment=8192)
+    assert plan.stats()["items"] > 40
+    plan.launch()
+    mtgp.synchronize()
+    plan.commit()
+    for k, (buf, dt, count) in enumerate(bufs):
+        assert_same(buf.to_host(dt, count), expect[k], "request %d" % k)
+    for g in range(len(specs)):
+        assert_same(gs.state(g), windows[g][1], "state %d" % g)
