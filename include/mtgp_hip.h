@@ -138,6 +138,13 @@ mtgp_plan_t *mtgp_plan_create(mtgp_genset_t *set, const mtgp_request_t *reqs, in
 int mtgp_plan_launch(mtgp_plan_t *plan, void *stream);
 int mtgp_plan_commit(mtgp_plan_t *plan);
 void mtgp_plan_destroy(mtgp_plan_t *plan);
+/* Tuning knobs (the autotuner's): step layout (0 = four consecutive
+ * numbers per lane, 128-bit stores; 1 = lane-strided, conflict-free LDS)
+ * and resident workgroups per CU; -1 leaves a knob unchanged. */
+int mtgp_plan_configure(mtgp_plan_t *plan, int layout, int blocks_per_cu);
+/* Enqueues one part of a plan: 0 = the start-state jump launches, 1 = the
+ * generation launches (part 1 reads what part 0 wrote: order them). */
+int mtgp_plan_launch_part(mtgp_plan_t *plan, int part, void *stream);
 /* Work items, kernel launches, and host planning time of a plan. */
 int mtgp_plan_stats(const mtgp_plan_t *plan, int *items, int *launches, double *plan_ms);
 /* plan + launch + commit + synchronize. */

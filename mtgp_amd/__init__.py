@@ -134,6 +134,8 @@ _sig("mtgp_plan_create", vp, vp, C.POINTER(Request), i32, u64)
 _sig("mtgp_plan_launch", i32, vp, vp)
 _sig("mtgp_plan_commit", i32, vp)
 _sig("mtgp_plan_destroy", None, vp)
+_sig("mtgp_plan_configure", i32, vp, i32, i32)
+_sig("mtgp_plan_launch_part", i32, vp, i32, vp)
 _sig("mtgp_plan_stats", i32, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double))
 _sig("mtgp_genset_generate", i32, vp, C.POINTER(Request), i32, vp)
 
@@ -414,8 +416,14 @@ class Plan:
     def launch(self, stream=None):
         _check(_lib.mtgp_plan_launch(self.h, stream.handle if stream else None), "plan_launch")
 
+    def launch_part(self, part, stream=None):
+        _check(_lib.mtgp_plan_launch_part(self.h, part, stream.handle if stream else None), "plan_launch_part")
+
     def commit(self):
         _check(_lib.mtgp_plan_commit(self.h), "plan_commit")
+
+    def configure(self, layout=-1, blocks_per_cu=-1):
+        _check(_lib.mtgp_plan_configure(self.h, layout, blocks_per_cu), "plan_configure")
 
     def stats(self):
         it, ln, ms = C.c_int(), C.c_int(), C.c_double()

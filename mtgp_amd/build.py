@@ -1,4 +1,4 @@
-"""Builds libmtgp_amd.so (gfx950) in-tree: python -m mtgp_amd.build
+"""Builds libmtgp_amd.so (gfx950) in-tree: python mtgp_amd/build.py
 
 Host C++ (parameter tables, init family, GF(2)[x]) is compiled with g++,
 the runtime and kernels with hipcc for gfx950, and everything is linked
@@ -65,7 +65,7 @@ def build(verbose=False, jobs=4):
         objs.append(o)
     for s in KERNEL_SOURCES:
         o = os.path.join(OUT, s + ".o")
-        cmds.append([hipcc] + common + ["--offload-arch=" + ARCH] + NO_PK_FP32 +
+        cmds.append([hipcc] + common + ["-std=c++20", "--offload-arch=" + ARCH] + NO_PK_FP32 +
                     ["-c", os.path.join(SRC, s), "-o", o])
         objs.append(o)
     procs = []
@@ -74,6 +74,16 @@ def build(verbose=False, jobs=4):
         procs = [ex.submit(_run, c, verbose) for c in cmds]
         for p in procs:
             p.result()
+    # Device assembly of the kernels, scanned for the SCC select hazard.
+    asm = os.path.join(OUT, "kernels-gfx950.s")
+    _run([hipcc] + common + ["-std=c++20", "--offload-arch=" + ARCH, "--cuda-device-only", "-S"] + NO_PK_FP32 +
+         [os.path.join(SRC, "kernels.hip"), "-o", asm], verbose)
+    sys.path.insert(0, HERE)
+    import isa_check
+    bad = isa_check.scan(open(asm).read())
+    if bad:
+        raise RuntimeError("kernels.hip: s_cselect after a carry/borrow write of SCC at %s" %
+                           ", ".join("line %d" % b[0] for b in bad))
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, LIB)

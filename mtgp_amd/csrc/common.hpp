@@ -30,6 +30,25 @@ struct alignas(16) Item {
 };
 static_assert(sizeof(Item) == 64, "Item layout");
 
+// Batched jump: nb (<= kJumpBatch) jumps of one base window, each written
+// to states_out + b * state_bytes.  The coefficients come pre-arranged by
+// chunk and residue: masks[(c * 64 + rho) * kJumpBatch + b] holds, in bit q,
+// the coefficient of x^(c * kJumpChunk + 64 q + rho) of jump b.
+constexpr int kJumpBatch = 4;
+constexpr int kJumpSteps = 4;                  // 256-word steps per chunk
+constexpr int kJumpChunk = kJumpSteps * 256;   // positions per chunk
+struct alignas(16) JumpItem {
+    uint64_t state_in;
+    uint64_t masks;       // const uint16_t *
+    uint64_t states_out;
+    int32_t nb;
+    int32_t nchunks;
+    int32_t param;
+    int32_t state_bytes;
+    uint64_t pad[3];
+};
+static_assert(sizeof(JumpItem) == 64, "JumpItem layout");
+
 // Expanded parameter record as the kernels read it.  For MTGP64 the
 // tables hold the high 32 bits of each entry (all low halves are zero).
 struct alignas(16) DevParam {

@@ -224,6 +224,26 @@ Poly Modulus::reduce(const Poly &c) const {
     return r;
 }
 
+Poly Modulus::reduce_any(const Poly &c) const {
+    // Fold the top d bits down repeatedly: c = hi * x^k + lo with
+    // deg(hi * x^k) < 2d handled by reduce().
+    Poly r = c;
+    long dc = degree(r);
+    while (dc >= 2 * d_) {
+        // split off bits [dc - 2d + 1, dc]: reduce that window, put it back
+        const long lo = dc - 2 * d_ + 1;
+        Poly top = extract(r, lo, 2 * d_);
+        Poly red = reduce(top);  // deg < d
+        // r = (r without the window) + red * x^lo
+        for (long i = lo; i <= dc; i++)
+            if (get_bit(r, i))
+                r[i >> 6] ^= 1ull << (i & 63);
+        xor_shifted(r, red, lo);
+        dc = degree(r);
+    }
+    return reduce(r);
+}
+
 Poly Modulus::mulmod(const Poly &a, const Poly &b) const { return reduce(multiply(a, b)); }
 Poly Modulus::sqrmod(const Poly &a) const { return reduce(square(a)); }
 

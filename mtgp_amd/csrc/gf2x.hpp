@@ -29,6 +29,7 @@ class Modulus {
     int words() const { return w_; }            // words of a reduced value
     const Poly &poly() const { return m_; }
     Poly reduce(const Poly &c) const;           // any c with deg c < 2d
+    Poly reduce_any(const Poly &c) const;       // any degree
     Poly mulmod(const Poly &a, const Poly &b) const;
     Poly sqrmod(const Poly &a) const;
     Poly mulx(const Poly &a) const;             // a * x mod m

@@ -92,7 +92,7 @@ __device__ __forceinline__ uint32_t out32(const uint32_t *tab, uint32_t r, uint3
 
 // One 256-word step from window base b (b % 4 == 0).  With OUT, writes the
 // m (<= 256) leading outputs to dst (16-byte aligned).
-template <int KIND, bool OUT>
+template <int KIND, bool OUT, bool FULL = false>
 __device__ __forceinline__ void step32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b,
                                        gu32 *dst, uint32_t m) {
     const int l = threadIdx.x;
@@ -117,7 +117,7 @@ __device__ __forceinline__ void step32(uint32_t *buf, const uint32_t *tab, const
         o.z = out32<KIND>(tab, r2, t2);
         o.w = out32<KIND>(tab, r3, t3);
         const uint32_t q = 4 * l;
-        if (q + 4 <= m) {
+        if (FULL || q + 4 <= m) {
             *reinterpret_cast<gu4 *>(dst + q) = v4u{o.x, o.y, o.z, o.w};
         } else if (q < m) {
             dst[q] = o.x;
@@ -125,6 +125,40 @@ __device__ __forceinline__ void step32(uint32_t *buf, const uint32_t *tab, const
                 dst[q + 1] = o.y;
             if (q + 2 < m)
                 dst[q + 2] = o.z;
+        }
+    }
+}
+
+// Lane-strided variant of step32: lane l computes words l + 64i (i < 4),
+// so every LDS access of the step is lane-consecutive (no bank conflicts)
+// and every store instruction writes 256 contiguous bytes.  No alignment
+// requirement on b or dst.
+template <int KIND, bool OUT, bool FULL = false>
+__device__ __forceinline__ void step32s(uint32_t *buf, const uint32_t *tab, const P32 &p, int b,
+                                        gu32 *dst, uint32_t m) {
+    const int l = threadIdx.x;
+    const uint32_t *xp = buf + b + l;
+    const uint32_t *yp = xp + p.pos - 1;
+    uint32_t r[4], t[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        t[i] = yp[kWave * i];
+        r[i] = rec32(p, tab, xp[kWave * i], xp[kWave * i + 1], yp[kWave * i + 1]);
+    }
+    uint32_t *wp = buf + b + p.n + l;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        wp[kWave * i] = r[i];
+    if (OUT) {
+        uint32_t o[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            o[i] = out32<KIND>(tab, r[i], t[i]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t q = (uint32_t)(l + kWave * i);
+            if (FULL || q < m)
+                dst[q] = o[i];
         }
     }
 }
@@ -144,6 +178,51 @@ __device__ __forceinline__ void relocate32(uint32_t *buf, int b, int n) {
         if (l + kWave * m < n)
             buf[l + kWave * m] = v[m];
     wave_sync();
+}
+
+// Folds the chunk of positions [a, a + 256*steps) (clipped to `total`) of
+// the jump polynomial into acc: acc[m] ^= w[k + l + 64m] for every set
+// coefficient k, with buf[i] = w[a + i].  Positions are visited by residue
+// rho = k mod 64: for a fixed rho every set coefficient k = a + 64q + rho
+// needs the same lane column buf[rho + l + 64r] (r = q + m), which each lane
+// loads once into registers; the XORs then read registers only.  This
+// halves the LDS traffic of visiting the coefficients one by one.
+template <int NJ, int K>
+__device__ __forceinline__ void fold32(const uint32_t *buf, const gu32 *__restrict__ poly, int a, int total,
+                                       int steps, uint32_t (&acc)[NJ]) {
+    constexpr int QMAX = 4 * K;  // 64-position groups per chunk
+    const int l = threadIdx.x;
+    const int end = a + steps * kStep < total ? a + steps * kStep : total;
+    const int nw = (end - a) / 32;  // coefficient words in this chunk
+    uint32_t cw[2 * QMAX];
+#pragma unroll
+    for (int i = 0; i < 2 * QMAX; i++)
+        cw[i] = i < nw ? uniform(poly[a / 32 + i]) : 0u;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        for (int r0 = 0; r0 < 32; r0++) {
+            const int rho = 32 * half + r0;
+            uint32_t any = 0;
+#pragma unroll
+            for (int q = 0; q < QMAX; q++)
+                any |= cw[2 * q + half] >> r0 & 1u;
+            if (!any)
+                continue;
+            uint32_t R[QMAX + NJ];
+            const uint32_t *col = buf + rho + l;
+#pragma unroll
+            for (int r = 0; r < QMAX + NJ; r++)
+                R[r] = col[kWave * r];
+#pragma unroll
+            for (int q = 0; q < QMAX; q++) {
+                if (cw[2 * q + half] >> r0 & 1u) {
+#pragma unroll
+                    for (int m = 0; m < NJ; m++)
+                        acc[m] ^= R[q + m];
+                }
+            }
+        }
+    }
 }
 
 // Window after jumping: acc[m] = new[l + 64m] (see file header).
@@ -167,18 +246,7 @@ __device__ void jump32(uint32_t *buf, const uint32_t *tab, const P32 &p, const g
         }
         // buf now holds w[a .. a + n + 256*steps); fold positions
         // [a, a + 256*steps) with set coefficients into acc.
-        const int qend = (a + steps * kStep < total ? a + steps * kStep : total) / 32;
-        for (int q = a / 32; q < qend; q++) {
-            uint32_t cw = uniform(poly[q]);
-            while (cw) {
-                const int bit = __builtin_ctz(cw);
-                cw &= cw - 1;
-                const uint32_t *src = buf + (32 * q + bit - a) + l;
-#pragma unroll
-                for (int m = 0; m < NJ; m++)
-                    acc[m] ^= src[kWave * m];
-            }
-        }
+        fold32<NJ, K>(buf, poly, a, total, steps, acc);
         a += steps * kStep;
         if (a < total)
             relocate32<NJ>(buf, steps * kStep, p.n);
@@ -186,7 +254,7 @@ __device__ void jump32(uint32_t *buf, const uint32_t *tab, const P32 &p, const g
     wave_sync();
 }
 
-template <int KIND, int NJ, int NBUF>
+template <int KIND, int NJ, int NBUF, int LAYOUT>
 __device__ void generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int o, int h,
                            gu32 *__restrict__ out, uint64_t count, gu32 *__restrict__ state_out) {
     const int l = threadIdx.x;
@@ -201,31 +269,48 @@ __device__ void generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int
         }
         wave_sync();
     }
+    // count - h numbers: whole steps, then one partial step.  (Written
+    // without a select on the 64-bit remainder.)
     int b = o + h;
-    uint64_t done = (uint64_t)h;
-    while (done < count) {
+    gu32 *dst = out + h;
+    const uint64_t rest = count - (uint64_t)h;
+    for (uint64_t s = rest / kStep; s > 0; s--) {
         if (b + p.n + kStep > NBUF) {
             relocate32<NJ>(buf, b, p.n);
             b = 0;
         }
-        const uint64_t left = count - done;
-        const uint32_t m = left < (uint64_t)kStep ? (uint32_t)left : (uint32_t)kStep;
-        step32<KIND, true>(buf, tab, p, b, out + done, m);
+        if (LAYOUT == 0)
+            step32<KIND, true, true>(buf, tab, p, b, dst, kStep);
+        else
+            step32s<KIND, true, true>(buf, tab, p, b, dst, kStep);
+        wave_sync();
+        b += kStep;
+        dst += kStep;
+    }
+    const uint32_t m = (uint32_t)(rest % kStep);
+    if (m) {
+        if (b + p.n + kStep > NBUF) {
+            relocate32<NJ>(buf, b, p.n);
+            b = 0;
+        }
+        if (LAYOUT == 0)
+            step32<KIND, true>(buf, tab, p, b, dst, m);
+        else
+            step32s<KIND, true>(buf, tab, p, b, dst, m);
         wave_sync();
         b += (int)m;
-        done += m;
     }
     if (state_out)
         for (int j = l; j < p.n; j += kWave)
             state_out[j] = buf[b + j];
 }
 
-template <int NMAX, int K>
+template <int NMAX, int K, int LAYOUT>
 __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict__ items, int n_items,
                                                           const DevParam *__restrict__ params,
                                                           uint32_t *__restrict__ queue) {
     constexpr int NJ = (NMAX + kWave - 1) / kWave;
-    constexpr int NBUF = K * kStep + kWave * NJ + 8;
+    constexpr int NBUF = K * kStep + kWave * NJ + kWave;
     __shared__ __attribute__((aligned(16))) uint32_t buf[NBUF];
     __shared__ uint32_t tab[48];
     const int l = threadIdx.x;
@@ -245,7 +330,7 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
         gu32 *out = reinterpret_cast<gu32 *>(I.out);
         gu32 *state_out = reinterpret_cast<gu32 *>(I.state_out);
         const gu32 *state_in = reinterpret_cast<const gu32 *>(I.state_in);
-        uint32_t h = (uint32_t)((16u - (uint32_t)(I.out & 15)) & 15) >> 2;
+        uint32_t h = LAYOUT == 0 ? (uint32_t)((16u - (uint32_t)(I.out & 15)) & 15) >> 2 : 0u;
         if ((uint64_t)h > count)
             h = (uint32_t)count;
         const int o = (int)((4u - h) & 3u);
@@ -264,11 +349,11 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
         }
         wave_sync();
         switch (I.kind) {
-        case 0: generate32<0, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
-        case 1: generate32<1, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
-        case 2: generate32<2, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
-        case 3: generate32<3, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
-        default: generate32<4, NJ, NBUF>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        case 0: generate32<0, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        case 1: generate32<1, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        case 2: generate32<2, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        case 3: generate32<3, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        default: generate32<4, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
         }
         wave_sync();
         uint32_t next = 0;
@@ -337,7 +422,7 @@ __device__ __forceinline__ void out64(const uint32_t *tab, uint32_t rh, uint32_t
     ol = (uint32_t)__double2loint(v);
 }
 
-template <int KIND, bool OUT>
+template <int KIND, bool OUT, bool FULL = false>
 __device__ __forceinline__ void step64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b,
                                        gu32 *dst, uint32_t m) {
     const int l = threadIdx.x;
@@ -371,7 +456,7 @@ __device__ __forceinline__ void step64(uint32_t *bh, uint32_t *bl, const uint32_
         out64<KIND>(tab, rh3, rl3, tl3, o1.w, o1.z);
         const uint32_t e = 4 * l;  // first output element of this lane
         gu4 *d4 = reinterpret_cast<gu4 *>(dst + 2 * e);
-        if (e + 4 <= m) {
+        if (FULL || e + 4 <= m) {
             d4[0] = v4u{o0.x, o0.y, o0.z, o0.w};
             d4[1] = v4u{o1.x, o1.y, o1.z, o1.w};
         } else if (e < m) {
@@ -381,6 +466,42 @@ __device__ __forceinline__ void step64(uint32_t *bh, uint32_t *bl, const uint32_
                 d2[1] = v2u{o0.z, o0.w};
             if (e + 2 < m)
                 d2[2] = v2u{o1.x, o1.y};
+        }
+    }
+}
+
+// Lane-strided variant of step64 (see step32s); outputs are 8-byte stores,
+// 512 contiguous bytes per instruction.
+template <int KIND, bool OUT, bool FULL = false>
+__device__ __forceinline__ void step64s(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b,
+                                        gu32 *dst, uint32_t m) {
+    const int l = threadIdx.x;
+    const int q = b + l;
+    const int yq = q + p.pos - 1;
+    uint32_t rh[4], rl[4], tl[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int x = q + kWave * i, y = yq + kWave * i;
+        tl[i] = bl[y];
+        rec64(p, tab, bh[x], bl[x], bh[x + 1], bl[x + 1], bh[y + 1], bl[y + 1], rh[i], rl[i]);
+    }
+    const int w = q + p.n;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        bh[w + kWave * i] = rh[i];
+        bl[w + kWave * i] = rl[i];
+    }
+    if (OUT) {
+        gu2 *d2 = reinterpret_cast<gu2 *>(dst);
+        uint32_t oh[4], ol[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            out64<KIND>(tab, rh[i], rl[i], tl[i], oh[i], ol[i]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t e = (uint32_t)(l + kWave * i);
+            if (FULL || e < m)
+                d2[e] = v2u{ol[i], oh[i]};
         }
     }
 }
@@ -405,6 +526,48 @@ __device__ __forceinline__ void relocate64(uint32_t *bh, uint32_t *bl, int b, in
     wave_sync();
 }
 
+// fold32 for the two 32-bit planes of MTGP64 words.
+template <int NJ, int K>
+__device__ __forceinline__ void fold64(const uint32_t *bh, const uint32_t *bl, const gu32 *__restrict__ poly, int a,
+                                       int total, int steps, uint32_t (&ah)[NJ], uint32_t (&al)[NJ]) {
+    constexpr int QMAX = 4 * K;
+    const int l = threadIdx.x;
+    const int end = a + steps * kStep < total ? a + steps * kStep : total;
+    const int nw = (end - a) / 32;  // coefficient words in this chunk
+    uint32_t cw[2 * QMAX];
+#pragma unroll
+    for (int i = 0; i < 2 * QMAX; i++)
+        cw[i] = i < nw ? uniform(poly[a / 32 + i]) : 0u;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        for (int r0 = 0; r0 < 32; r0++) {
+            const int rho = 32 * half + r0;
+            uint32_t any = 0;
+#pragma unroll
+            for (int q = 0; q < QMAX; q++)
+                any |= cw[2 * q + half] >> r0 & 1u;
+            if (!any)
+                continue;
+            uint32_t RH[QMAX + NJ], RL[QMAX + NJ];
+#pragma unroll
+            for (int r = 0; r < QMAX + NJ; r++) {
+                RH[r] = bh[rho + l + kWave * r];
+                RL[r] = bl[rho + l + kWave * r];
+            }
+#pragma unroll
+            for (int q = 0; q < QMAX; q++) {
+                if (cw[2 * q + half] >> r0 & 1u) {
+#pragma unroll
+                    for (int m = 0; m < NJ; m++) {
+                        ah[m] ^= RH[q + m];
+                        al[m] ^= RL[q + m];
+                    }
+                }
+            }
+        }
+    }
+}
+
 template <int NJ, int K, int NBUF>
 __device__ void jump64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p,
                        const gu64 *__restrict__ in, const gu32 *__restrict__ poly, int poly_words,
@@ -427,20 +590,7 @@ __device__ void jump64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P6
             step64<0, false>(bh, bl, tab, p, s * kStep, nullptr, 0);
             wave_sync();
         }
-        const int qend = (a + steps * kStep < total ? a + steps * kStep : total) / 32;
-        for (int q = a / 32; q < qend; q++) {
-            uint32_t cw = uniform(poly[q]);
-            while (cw) {
-                const int bit = __builtin_ctz(cw);
-                cw &= cw - 1;
-                const int off = (32 * q + bit - a) + l;
-#pragma unroll
-                for (int m = 0; m < NJ; m++) {
-                    ah[m] ^= bh[off + kWave * m];
-                    al[m] ^= bl[off + kWave * m];
-                }
-            }
-        }
+        fold64<NJ, K>(bh, bl, poly, a, total, steps, ah, al);
         a += steps * kStep;
         if (a < total)
             relocate64<NJ>(bh, bl, steps * kStep, p.n);
@@ -448,7 +598,7 @@ __device__ void jump64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P6
     wave_sync();
 }
 
-template <int KIND, int NJ, int NBUF>
+template <int KIND, int NJ, int NBUF, int LAYOUT>
 __device__ void generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int o, int h,
                            gu32 *__restrict__ out, uint64_t count, gu64 *__restrict__ state_out) {
     const int l = threadIdx.x;
@@ -465,30 +615,45 @@ __device__ void generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, cons
         wave_sync();
     }
     int b = o + h;
-    uint64_t done = (uint64_t)h;
-    while (done < count) {
+    gu32 *dst = out + 2 * h;
+    const uint64_t rest = count - (uint64_t)h;
+    for (uint64_t s = rest / kStep; s > 0; s--) {
         if (b + p.n + kStep > NBUF) {
             relocate64<NJ>(bh, bl, b, p.n);
             b = 0;
         }
-        const uint64_t left = count - done;
-        const uint32_t m = left < (uint64_t)kStep ? (uint32_t)left : (uint32_t)kStep;
-        step64<KIND, true>(bh, bl, tab, p, b, out + 2 * done, m);
+        if (LAYOUT == 0)
+            step64<KIND, true, true>(bh, bl, tab, p, b, dst, kStep);
+        else
+            step64s<KIND, true, true>(bh, bl, tab, p, b, dst, kStep);
+        wave_sync();
+        b += kStep;
+        dst += 2 * kStep;
+    }
+    const uint32_t m = (uint32_t)(rest % kStep);
+    if (m) {
+        if (b + p.n + kStep > NBUF) {
+            relocate64<NJ>(bh, bl, b, p.n);
+            b = 0;
+        }
+        if (LAYOUT == 0)
+            step64<KIND, true>(bh, bl, tab, p, b, dst, m);
+        else
+            step64s<KIND, true>(bh, bl, tab, p, b, dst, m);
         wave_sync();
         b += (int)m;
-        done += m;
     }
     if (state_out)
         for (int j = l; j < p.n; j += kWave)
             state_out[j] = ((uint64_t)bh[b + j] << 32) | bl[b + j];
 }
 
-template <int NMAX, int K>
+template <int NMAX, int K, int LAYOUT>
 __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict__ items, int n_items,
                                                           const DevParam *__restrict__ params,
                                                           uint32_t *__restrict__ queue) {
     constexpr int NJ = (NMAX + kWave - 1) / kWave;
-    constexpr int NBUF = K * kStep + kWave * NJ + 8;
+    constexpr int NBUF = K * kStep + kWave * NJ + kWave;
     __shared__ __attribute__((aligned(16))) uint32_t bh[NBUF];
     __shared__ __attribute__((aligned(16))) uint32_t bl[NBUF];
     __shared__ uint32_t tab[48];
@@ -510,7 +675,7 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         gu32 *out = reinterpret_cast<gu32 *>(I.out);
         gu64 *state_out = reinterpret_cast<gu64 *>(I.state_out);
         const gu64 *state_in = reinterpret_cast<const gu64 *>(I.state_in);
-        uint32_t h = (uint32_t)((I.out & 15) ? 1u : 0u);  // 8-byte outputs
+        uint32_t h = (LAYOUT == 0 && (I.out & 15)) ? 1u : 0u;  // 8-byte outputs
         if ((uint64_t)h > count)
             h = (uint32_t)count;
         const int o = (int)((4u - h) & 3u);
@@ -534,11 +699,11 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         }
         wave_sync();
         switch (I.kind) {
-        case 0: generate64<0, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
-        case 1: generate64<1, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
-        case 2: generate64<2, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
-        case 3: generate64<3, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
-        default: generate64<4, NJ, NBUF>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        case 0: generate64<0, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        case 1: generate64<1, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        case 2: generate64<2, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        case 3: generate64<3, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        default: generate64<4, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
         }
         wave_sync();
         uint32_t next = 0;
@@ -555,20 +720,223 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
     }
 }
 
-// Steps per LDS relocation, by class.
+// ------------------------------------------------------ batched jump kernels
+// One workgroup takes a JumpItem: the base window's sequence is generated
+// chunk by chunk in LDS (kJumpSteps steps per chunk) and every chunk is
+// folded into kJumpBatch accumulators at once.  For each residue rho the
+// lane column buf[rho + l + 64r] is loaded into registers once and shared
+// by all jumps of the batch, so LDS traffic per jump drops by the batch
+// size; the host supplies each jump's coefficients as per-residue masks.
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
+}
+
+typedef GMEM uint64_t gu64c;
+
+template <int NMAX>
+__global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restrict__ items, int n_items,
+                                                         const DevParam *__restrict__ params,
+                                                         uint32_t *__restrict__ queue) {
+    constexpr int K = kJumpSteps;
+    constexpr int NJ = (NMAX + kWave - 1) / kWave;
+    constexpr int QMAX = 4 * K;
+    constexpr int B = kJumpBatch;
+    constexpr int NBUF = K * kStep + kWave * NJ + kWave;
+    static_assert(B * 16 == 64 && QMAX == 16, "one 64-bit mask word per residue");
+    __shared__ __attribute__((aligned(16))) uint32_t buf[NBUF];
+    __shared__ uint32_t tab[48];
+    const int l = threadIdx.x;
+    uint32_t it = blockIdx.x;
+    while (it < (uint32_t)n_items) {
+        const JumpItem &I = items[it];
+        const DevParam &D = params[I.param];
+        P32 p;
+        p.n = D.n;
+        p.pos = D.pos;
+        p.sh1 = D.sh1;
+        p.sh2 = D.sh2;
+        p.mask = D.mask_lo;
+        if (l < 48)
+            tab[l] = D.tbl[l];
+        const gu32 *in = reinterpret_cast<const gu32 *>(I.state_in);
+        const gu64c *mk = reinterpret_cast<const gu64c *>(I.masks);
+        wave_sync();
+        for (int j = l; j < p.n; j += kWave)
+            buf[j] = in[j];
+        wave_sync();
+        uint32_t acc[B][NJ];
+#pragma unroll
+        for (int b = 0; b < B; b++)
+#pragma unroll
+            for (int m = 0; m < NJ; m++)
+                acc[b][m] = 0;
+        for (int c = 0; c < I.nchunks; c++) {
+            for (int s = 0; s < K; s++) {
+                step32s<0, false>(buf, tab, p, s * kStep, nullptr, 0);
+                wave_sync();
+            }
+            for (int rho = 0; rho < kWave; rho++) {
+                const uint64_t mm = uniform64(mk[c * kWave + rho]);
+                if (!mm)
+                    continue;
+                uint32_t R[QMAX + NJ];
+                const uint32_t *col = buf + rho + l;
+#pragma unroll
+                for (int r = 0; r < QMAX + NJ; r++)
+                    R[r] = col[kWave * r];
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    const uint32_t mask = (uint32_t)(mm >> (16 * b)) & 0xffffu;
+                    if (!mask)
+                        continue;
+#pragma unroll
+                    for (int q = 0; q < QMAX; q++)
+                        if (mask >> q & 1u) {
+#pragma unroll
+                            for (int m = 0; m < NJ; m++)
+                                acc[b][m] ^= R[q + m];
+                        }
+                }
+            }
+            relocate32<NJ>(buf, K * kStep, p.n);
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if (b < I.nb) {
+                gu32 *o = reinterpret_cast<gu32 *>(I.states_out + (uint64_t)b * (uint64_t)I.state_bytes);
+#pragma unroll
+                for (int m = 0; m < NJ; m++)
+                    if (l + kWave * m < p.n)
+                        o[l + kWave * m] = acc[b][m];
+            }
+        }
+        wave_sync();
+        uint32_t next = 0;
+        if (l == 0)
+            next = atomicAdd(&queue[0], 1u);
+        it = gridDim.x + uniform(__shfl(next, 0));
+    }
+    if (l == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restrict__ items, int n_items,
+                                                         const DevParam *__restrict__ params,
+                                                         uint32_t *__restrict__ queue) {
+    constexpr int K = kJumpSteps;
+    constexpr int NJ = (NMAX + kWave - 1) / kWave;
+    constexpr int QMAX = 4 * K;
+    constexpr int B = kJumpBatch;
+    constexpr int NBUF = K * kStep + kWave * NJ + kWave;
+    __shared__ __attribute__((aligned(16))) uint32_t bh[NBUF];
+    __shared__ __attribute__((aligned(16))) uint32_t bl[NBUF];
+    __shared__ uint32_t tab[48];
+    const int l = threadIdx.x;
+    uint32_t it = blockIdx.x;
+    while (it < (uint32_t)n_items) {
+        const JumpItem &I = items[it];
+        const DevParam &D = params[I.param];
+        P64 p;
+        p.n = D.n;
+        p.pos = D.pos;
+        p.sh1 = D.sh1;
+        p.sh2 = D.sh2;
+        p.mask_hi = D.mask_hi;
+        p.mask_lo = D.mask_lo;
+        if (l < 48)
+            tab[l] = D.tbl[l];
+        const gu64 *in = reinterpret_cast<const gu64 *>(I.state_in);
+        const gu64c *mk = reinterpret_cast<const gu64c *>(I.masks);
+        wave_sync();
+        for (int j = l; j < p.n; j += kWave) {
+            const uint64_t v = in[j];
+            bh[j] = (uint32_t)(v >> 32);
+            bl[j] = (uint32_t)v;
+        }
+        wave_sync();
+        uint32_t ah[B][NJ], al[B][NJ];
+#pragma unroll
+        for (int b = 0; b < B; b++)
+#pragma unroll
+            for (int m = 0; m < NJ; m++)
+                ah[b][m] = al[b][m] = 0;
+        for (int c = 0; c < I.nchunks; c++) {
+            for (int s = 0; s < K; s++) {
+                step64s<0, false>(bh, bl, tab, p, s * kStep, nullptr, 0);
+                wave_sync();
+            }
+            for (int rho = 0; rho < kWave; rho++) {
+                const uint64_t mm = uniform64(mk[c * kWave + rho]);
+                if (!mm)
+                    continue;
+                uint32_t RH[QMAX + NJ], RL[QMAX + NJ];
+#pragma unroll
+                for (int r = 0; r < QMAX + NJ; r++) {
+                    RH[r] = bh[rho + l + kWave * r];
+                    RL[r] = bl[rho + l + kWave * r];
+                }
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    const uint32_t mask = (uint32_t)(mm >> (16 * b)) & 0xffffu;
+                    if (!mask)
+                        continue;
+#pragma unroll
+                    for (int q = 0; q < QMAX; q++)
+                        if (mask >> q & 1u) {
+#pragma unroll
+                            for (int m = 0; m < NJ; m++) {
+                                ah[b][m] ^= RH[q + m];
+                                al[b][m] ^= RL[q + m];
+                            }
+                        }
+                }
+            }
+            relocate64<NJ>(bh, bl, K * kStep, p.n);
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if (b < I.nb) {
+                gu64 *o = reinterpret_cast<gu64 *>(I.states_out + (uint64_t)b * (uint64_t)I.state_bytes);
+#pragma unroll
+                for (int m = 0; m < NJ; m++)
+                    if (l + kWave * m < p.n)
+                        o[l + kWave * m] = ((uint64_t)ah[b][m] << 32) | al[b][m];
+            }
+        }
+        wave_sync();
+        uint32_t next = 0;
+        if (l == 0)
+            next = atomicAdd(&queue[0], 1u);
+        it = gridDim.x + uniform(__shfl(next, 0));
+    }
+    if (l == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
+// Steps per LDS relocation.
 constexpr int kK = 3;
 
-template <int B, int NMAX>
+template <int B, int NMAX, int LAYOUT>
 struct KernelOf;
-template <int NMAX>
-struct KernelOf<32, NMAX> {
-    static constexpr auto fn = mtgp32_items_kernel<NMAX, kK>;
-    static constexpr int lds = 4 * ((kK * kStep + kWave * ((NMAX + 63) / 64) + 8) + 48);
+template <int NMAX, int LAYOUT>
+struct KernelOf<32, NMAX, LAYOUT> {
+    static constexpr auto fn = mtgp32_items_kernel<NMAX, kK, LAYOUT>;
 };
-template <int NMAX>
-struct KernelOf<64, NMAX> {
-    static constexpr auto fn = mtgp64_items_kernel<NMAX, kK>;
-    static constexpr int lds = 4 * (2 * (kK * kStep + kWave * ((NMAX + 63) / 64) + 8) + 48);
+template <int NMAX, int LAYOUT>
+struct KernelOf<64, NMAX, LAYOUT> {
+    static constexpr auto fn = mtgp64_items_kernel<NMAX, kK, LAYOUT>;
 };
 
 template <class KO>
@@ -578,9 +946,18 @@ hipError_t launch_one(const Item *items, int n_items, const DevParam *params, ui
     return hipGetLastError();
 }
 
-template <class KO>
-const void *fn_of() {
-    return reinterpret_cast<const void *>(KO::fn);
+// Calls f.template operator()<KernelOf<...>>() for (bits, cls, layout).
+template <class F>
+auto with_kernel(int bits, int cls, int layout, F &&f) {
+#define MTGP_CASE(B, N, L) \
+    if (bits == B && N_OF(B, cls) == N && layout == L) return f.template operator()<KernelOf<B, N, L>>();
+#define N_OF(B, c) ((B) == 32 ? ((c) == 0 ? 351 : (c) == 1 ? 726 : 1391) : ((c) == 0 ? 363 : (c) == 1 ? 696 : 1727))
+    MTGP_CASE(32, 351, 0) MTGP_CASE(32, 351, 1) MTGP_CASE(32, 726, 0) MTGP_CASE(32, 726, 1)
+    MTGP_CASE(32, 1391, 0) MTGP_CASE(32, 1391, 1) MTGP_CASE(64, 363, 0) MTGP_CASE(64, 363, 1)
+    MTGP_CASE(64, 696, 0) MTGP_CASE(64, 696, 1) MTGP_CASE(64, 1727, 0) MTGP_CASE(64, 1727, 1)
+#undef MTGP_CASE
+#undef N_OF
+    return f.template operator()<KernelOf<64, 1727, 1>>();
 }
 
 }  // namespace
@@ -592,41 +969,34 @@ int kernel_class(int bits, int n) {
     return n <= 363 ? 0 : n <= 696 ? 1 : n <= 1727 ? 2 : -1;
 }
 
-const void *kernel_fn(int bits, int cls) {
-    if (bits == 32) {
-        switch (cls) {
-        case 0: return fn_of<KernelOf<32, 351>>();
-        case 1: return fn_of<KernelOf<32, 726>>();
-        default: return fn_of<KernelOf<32, 1391>>();
-        }
-    }
-    switch (cls) {
-    case 0: return fn_of<KernelOf<64, 363>>();
-    case 1: return fn_of<KernelOf<64, 696>>();
-    default: return fn_of<KernelOf<64, 1727>>();
-    }
+const void *kernel_fn(int bits, int cls, int layout) {
+    return with_kernel(bits, cls, layout, []<class KO>() { return reinterpret_cast<const void *>(KO::fn); });
 }
 
-int kernel_lds_bytes(int bits, int cls) {
-    if (bits == 32)
-        return cls == 0 ? KernelOf<32, 351>::lds : cls == 1 ? KernelOf<32, 726>::lds : KernelOf<32, 1391>::lds;
-    return cls == 0 ? KernelOf<64, 363>::lds : cls == 1 ? KernelOf<64, 696>::lds : KernelOf<64, 1727>::lds;
-}
-
-hipError_t launch_items(int bits, int cls, const Item *items, int n_items, const DevParam *params,
+hipError_t launch_items(int bits, int cls, int layout, const Item *items, int n_items, const DevParam *params,
                         uint32_t *queue, int grid, hipStream_t s) {
-    if (bits == 32) {
-        switch (cls) {
-        case 0: return launch_one<KernelOf<32, 351>>(items, n_items, params, queue, grid, s);
-        case 1: return launch_one<KernelOf<32, 726>>(items, n_items, params, queue, grid, s);
-        default: return launch_one<KernelOf<32, 1391>>(items, n_items, params, queue, grid, s);
-        }
-    }
-    switch (cls) {
-    case 0: return launch_one<KernelOf<64, 363>>(items, n_items, params, queue, grid, s);
-    case 1: return launch_one<KernelOf<64, 696>>(items, n_items, params, queue, grid, s);
-    default: return launch_one<KernelOf<64, 1727>>(items, n_items, params, queue, grid, s);
-    }
+    return with_kernel(bits, cls, layout, [&]<class KO>() { return launch_one<KO>(items, n_items, params, queue, grid, s); });
+}
+
+namespace {
+template <class F>
+auto with_jump_kernel(int bits, int cls, F &&f) {
+    if (bits == 32)
+        return cls == 0 ? f(mtgp32_jump_kernel<351>) : cls == 1 ? f(mtgp32_jump_kernel<726>) : f(mtgp32_jump_kernel<1391>);
+    return cls == 0 ? f(mtgp64_jump_kernel<363>) : cls == 1 ? f(mtgp64_jump_kernel<696>) : f(mtgp64_jump_kernel<1727>);
+}
+}  // namespace
+
+const void *jump_kernel_fn(int bits, int cls) {
+    return with_jump_kernel(bits, cls, [](auto fn) { return reinterpret_cast<const void *>(fn); });
+}
+
+hipError_t launch_jumps(int bits, int cls, const JumpItem *items, int n_items, const DevParam *params,
+                        uint32_t *queue, int grid, hipStream_t s) {
+    return with_jump_kernel(bits, cls, [&](auto fn) {
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(kWave), 0, s, items, n_items, params, queue);
+        return hipGetLastError();
+    });
 }
 
 }  // namespace mtgp

@@ -7,14 +7,21 @@ namespace mtgp {
 
 // Kernel class for a state of n words (-1 if none fits).
 int kernel_class(int bits, int n);
-// Static LDS bytes per workgroup of a class (one wave per workgroup).
-int kernel_lds_bytes(int bits, int cls);
+// Step layouts: 0 = each lane computes 4 consecutive words and stores them
+// with one 128-bit store; 1 = lane l computes words l + 64i (conflict-free
+// LDS, 256-byte coalesced dword stores).
+constexpr int kLayouts = 2;
 // Kernel entry (for occupancy queries).
-const void *kernel_fn(int bits, int cls);
+const void *kernel_fn(int bits, int cls, int layout);
 // Runs items[0, n_items) (device array, all of word size `bits` and class
 // <= cls) with `grid` persistent workgroups.  `queue` is a device pair of
 // uint32_t counters that is zero on entry and zero again on exit.
-hipError_t launch_items(int bits, int cls, const Item *items, int n_items, const DevParam *params,
+hipError_t launch_items(int bits, int cls, int layout, const Item *items, int n_items, const DevParam *params,
+                        uint32_t *queue, int grid, hipStream_t s);
+
+// Batched jump kernel (JumpItem, common.hpp).
+const void *jump_kernel_fn(int bits, int cls);
+hipError_t launch_jumps(int bits, int cls, const JumpItem *items, int n_items, const DevParam *params,
                         uint32_t *queue, int grid, hipStream_t s);
 
 }  // namespace mtgp

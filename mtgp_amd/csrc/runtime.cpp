@@ -78,12 +78,32 @@ Context &context() {
               "hipMemcpy params");
     c->n_params = kNumSets;
     for (int b = 0; b < 2; b++)
+        for (int cls = 0; cls < 3; cls++)
+            for (int lay = 0; lay < kLayouts; lay++) {
+                int nb = 0;
+                hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_fn(b ? 64 : 32, cls, lay), 64, 0),
+                          "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+                c->blocks_per_cu[b][cls][lay] = nb > 0 ? nb : 1;
+            }
+    for (int b = 0; b < 2; b++)
         for (int cls = 0; cls < 3; cls++) {
             int nb = 0;
-            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_fn(b ? 64 : 32, cls), 64, 0),
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, jump_kernel_fn(b ? 64 : 32, cls), 64, 0),
                       "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-            c->blocks_per_cu[b][cls] = nb > 0 ? nb : 1;
+            c->jump_blocks_per_cu[b][cls] = nb > 0 ? nb : 1;
         }
+    if (const char *e = getenv("MTGP_BLOCKS_PER_CU")) {
+        const int v = atoi(e);
+        if (v > 0)
+            c->default_bpc = v;
+    }
+    if (const char *e = getenv("MTGP_LAYOUT")) {
+        const int lay = atoi(e);
+        if (lay >= 0 && lay < kLayouts)
+            for (int b = 0; b < 2; b++)
+                for (int cls = 0; cls < 3; cls++)
+                    c->layout[b][cls] = lay;
+    }
     Context &ref = *c;
     g_ctx[dev] = std::move(c);
     return ref;
@@ -303,13 +323,41 @@ void parallel_for(int n, F f) {
 
 }  // namespace
 
-uint64_t auto_segment(const Context &c, uint64_t total_numbers) {
-    // Enough segments to fill the GPU (about 8 per CU) but no shorter than
-    // 2^20 numbers, so that the start-state jump (about mexp/2 XORs of the
-    // window) stays a small fraction of a segment's work.
-    const uint64_t target = (uint64_t)c.cus * 8;
-    uint64_t s = (total_numbers + target - 1) / target;
-    s = std::max<uint64_t>(s, 1ull << 20);
+int append_jump_masks(const std::vector<const std::vector<uint32_t> *> &polys, std::vector<uint64_t> &out) {
+    size_t ncoeff = 0;
+    for (const auto *w : polys)
+        ncoeff = std::max(ncoeff, 32 * w->size());
+    const int nchunks = std::max(1, (int)((ncoeff + kJumpChunk - 1) / kJumpChunk));
+    const size_t base = out.size();
+    out.resize(base + (size_t)nchunks * 64, 0);
+    for (size_t b = 0; b < polys.size(); b++) {
+        const std::vector<uint32_t> &w = *polys[b];
+        for (size_t i = 0; i < w.size(); i++) {
+            uint32_t v = w[i];
+            while (v) {
+                const int bit = __builtin_ctz(v);
+                v &= v - 1;
+                const long k = (long)(32 * i + bit);  // coefficient index
+                const long c = k / kJumpChunk, r = k % kJumpChunk;
+                const long q = r / 64, rho = r % 64;
+                if (c < nchunks)
+                    out[base + (size_t)(c * 64 + rho)] |= (uint64_t)1 << (16 * b + q);
+            }
+        }
+    }
+    return nchunks;
+}
+
+uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, int slots) {
+    // All segments of a launch run at once when there are no more of them
+    // than resident workgroups, and the launch then takes as long as one
+    // segment (plus its start-state jump).  So: as many equal segments as
+    // there are workgroup slots, leaving room for one partial segment per
+    // request.
+    (void)c;
+    const uint64_t usable = (uint64_t)std::max(1, slots - n_requests);
+    uint64_t s = (total_numbers + usable - 1) / usable;
+    s = std::max<uint64_t>(s, 4096);
     return (s + 255) & ~255ull;
 }
 
@@ -338,7 +386,16 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             throw Error("output buffer not aligned to its element size");
         all += r.count;
     }
-    const uint64_t S = segment ? segment : auto_segment(c, all);
+    int nonempty = 0;
+    for (const auto &r : reqs)
+        nonempty += r.count > 0;
+    // Resident workgroups of the default configuration of the largest class.
+    int slots = 0;
+    for (const auto &g : gens) {
+        const int b = g.bits == 64, cls = kernel_class(g.bits, g.n);
+        slots = std::max(slots, c.cus * std::min(c.default_bpc, c.blocks_per_cu[b][cls][c.layout[b][cls]]));
+    }
+    const uint64_t S = segment ? segment : auto_segment(c, all, nonempty, slots);
     for (int ri = 0; ri < (int)reqs.size(); ri++) {
         const auto &r = reqs[(size_t)ri];
         uint64_t &pos = total[(size_t)r.gen];
@@ -393,12 +450,49 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             out[k] = gf2::to_words32(cur, pw);
         }
     });
-    // Items, grouped by (word size, kernel class), longest first.
+    // Items, grouped by (word size, kernel class), longest first.  With
+    // jump_mode 1 the start states of pieces at off > 0 come from batched
+    // jump items (kJumpBatch per item, all from the generator's state)
+    // that write them to a state arena.
+    if (const char *e = getenv("MTGP_JUMP_MODE"))
+        jump_mode = atoi(e) ? 1 : 0;
     std::vector<Item> list;
     std::vector<int> group;
     std::vector<uint32_t> poly_host;
+    std::vector<uint64_t> mask_host;
+    std::vector<int> jgroup;
+    std::vector<size_t> arena_off;  // per piece with a jump (mode 1)
+    size_t arena_bytes = 0;
+    jitems.clear();
     for (int g = 0; g < G; g++) {
         const GenRef &gr = gens[(size_t)g];
+        const size_t sbytes = (size_t)gr.n * (size_t)(gr.bits / 8);
+        std::vector<size_t> slot_of_piece(pieces[(size_t)g].size(), (size_t)-1);
+        if (jump_mode == 1) {
+            std::vector<size_t> need;
+            for (size_t k = 0; k < pieces[(size_t)g].size(); k++)
+                if (pieces[(size_t)g][k].off > 0)
+                    need.push_back(k);
+            for (size_t i0 = 0; i0 < need.size(); i0 += kJumpBatch) {
+                const size_t nb = std::min<size_t>(kJumpBatch, need.size() - i0);
+                std::vector<const std::vector<uint32_t> *> ps;
+                for (size_t i = 0; i < nb; i++) {
+                    ps.push_back(&polys[(size_t)g][need[i0 + i]]);
+                    slot_of_piece[need[i0 + i]] = arena_bytes + i * sbytes;
+                }
+                JumpItem ji{};
+                ji.state_in = (uint64_t)(uintptr_t)gr.state_in;
+                ji.masks = mask_host.size() * 8;  // byte offset, fixed up below
+                ji.nchunks = append_jump_masks(ps, mask_host);
+                ji.states_out = arena_bytes;      // byte offset, fixed up below
+                ji.nb = (int32_t)nb;
+                ji.param = gr.slot;
+                ji.state_bytes = (int32_t)sbytes;
+                jitems.push_back(ji);
+                jgroup.push_back((gr.bits == 64 ? 3 : 0) + kernel_class(gr.bits, gr.n));
+                arena_bytes += ((nb * sbytes + 255) & ~(size_t)255);
+            }
+        }
         for (size_t k = 0; k < pieces[(size_t)g].size(); k++) {
             const Piece &p = pieces[(size_t)g][k];
             Item it{};
@@ -409,13 +503,17 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             it.state_in = (uint64_t)(uintptr_t)gr.state_in;
             it.state_out = (p.off + p.len == total[(size_t)g]) ? (uint64_t)(uintptr_t)gr.state_out : 0;
             it.param = gr.slot;
+            it.poly = UINT64_MAX;
             if (p.off > 0) {
-                const auto &pw = polys[(size_t)g][k];
-                it.poly = poly_host.size();  // word offset, fixed up below
-                it.poly_words = (int32_t)pw.size();
-                poly_host.insert(poly_host.end(), pw.begin(), pw.end());
-            } else {
-                it.poly = UINT64_MAX;
+                if (jump_mode == 1) {
+                    it.state_in = UINT64_MAX - 1;  // arena slot, fixed up below
+                    it.pad2 = slot_of_piece[k];
+                } else {
+                    const auto &pw = polys[(size_t)g][k];
+                    it.poly = poly_host.size();  // word offset, fixed up below
+                    it.poly_words = (int32_t)pw.size();
+                    poly_host.insert(poly_host.end(), pw.begin(), pw.end());
+                }
             }
             list.push_back(it);
             group.push_back((gr.bits == 64 ? 3 : 0) + kernel_class(gr.bits, gr.n));
@@ -440,11 +538,56 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         hip_check(hipMalloc((void **)&d_polys, poly_host.size() * 4), "hipMalloc polys");
         hip_check(hipMemcpy(d_polys, poly_host.data(), poly_host.size() * 4, hipMemcpyHostToDevice), "hipMemcpy polys");
     }
+    if (!mask_host.empty()) {
+        hip_check(hipMalloc((void **)&d_masks, mask_host.size() * 8), "hipMalloc masks");
+        hip_check(hipMemcpy(d_masks, mask_host.data(), mask_host.size() * 8, hipMemcpyHostToDevice), "hipMemcpy masks");
+        hip_check(hipMalloc((void **)&d_jstates, arena_bytes), "hipMalloc state arena");
+    }
+    for (auto &ji : jitems) {
+        ji.masks = (uint64_t)(uintptr_t)d_masks + ji.masks;
+        ji.states_out = (uint64_t)(uintptr_t)d_jstates + ji.states_out;
+    }
     items.clear();
     for (int i : order) {
         Item it = list[(size_t)i];
         it.poly = it.poly == UINT64_MAX ? 0 : (uint64_t)(uintptr_t)(d_polys + it.poly);
+        if (it.state_in == UINT64_MAX - 1) {
+            it.state_in = (uint64_t)(uintptr_t)d_jstates + it.pad2;
+            it.pad2 = 0;
+        }
         items.push_back(it);
+    }
+    // Jump launches, one per (word size, class), before the generation.
+    jlaunches.clear();
+    {
+        std::vector<int> jorder(jitems.size());
+        for (size_t i = 0; i < jorder.size(); i++)
+            jorder[i] = (int)i;
+        std::stable_sort(jorder.begin(), jorder.end(), [&](int a, int b) { return jgroup[(size_t)a] < jgroup[(size_t)b]; });
+        std::vector<JumpItem> sorted;
+        for (int i : jorder)
+            sorted.push_back(jitems[(size_t)i]);
+        jitems.swap(sorted);
+        for (size_t i = 0; i < jorder.size();) {
+            const int gi = jgroup[(size_t)jorder[i]];
+            size_t j = i;
+            while (j < jorder.size() && jgroup[(size_t)jorder[j]] == gi)
+                j++;
+            Launch L;
+            L.bits = gi >= 3 ? 64 : 32;
+            L.cls = gi % 3;
+            L.layout = 1;
+            L.first = (int)i;
+            L.count = (int)(j - i);
+            L.grid = std::min(L.count, c.cus * c.jump_blocks_per_cu[L.bits == 64][L.cls]);
+            jlaunches.push_back(L);
+            i = j;
+        }
+    }
+    if (!jitems.empty()) {
+        hip_check(hipMalloc((void **)&d_jitems, jitems.size() * sizeof(JumpItem)), "hipMalloc jump items");
+        hip_check(hipMemcpy(d_jitems, jitems.data(), jitems.size() * sizeof(JumpItem), hipMemcpyHostToDevice),
+                  "hipMemcpy jump items");
     }
     launches.clear();
     for (size_t i = 0; i < order.size();) {
@@ -455,9 +598,10 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         Launch L;
         L.bits = gi >= 3 ? 64 : 32;
         L.cls = gi % 3;
+        L.layout = c.layout[L.bits == 64][L.cls];
         L.first = (int)i;
         L.count = (int)(j - i);
-        const int cap = c.cus * c.blocks_per_cu[L.bits == 64][L.cls];
+        const int cap = c.cus * std::min(c.default_bpc, c.blocks_per_cu[L.bits == 64][L.cls][L.layout]);
         L.grid = std::min(L.count, cap);
         launches.push_back(L);
         i = j;
@@ -466,22 +610,58 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         hip_check(hipMalloc((void **)&d_items, items.size() * sizeof(Item)), "hipMalloc items");
         hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice),
                   "hipMemcpy items");
-        hip_check(hipMalloc((void **)&d_queue, launches.size() * 2 * sizeof(uint32_t)), "hipMalloc queue");
-        hip_check(hipMemset(d_queue, 0, launches.size() * 2 * sizeof(uint32_t)), "hipMemset queue");
+        const size_t nq = (launches.size() + jlaunches.size()) * 2 * sizeof(uint32_t);
+        hip_check(hipMalloc((void **)&d_queue, nq), "hipMalloc queue");
+        hip_check(hipMemset(d_queue, 0, nq), "hipMemset queue");
     }
     segment_used = S;
     plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void Plan::launch(hipStream_t s) {
-    for (size_t k = 0; k < launches.size(); k++) {
+    launch_part(0, s);
+    launch_part(1, s);
+}
+
+void Plan::launch_part(int part, hipStream_t s) {
+    const size_t jq = 2 * launches.size();
+    for (size_t k = 0; part == 0 && k < jlaunches.size(); k++) {
+        const Launch &L = jlaunches[k];
+        hip_check(launch_jumps(L.bits, L.cls, d_jitems + L.first, L.count, ctx->d_params, d_queue + jq + 2 * k, L.grid, s),
+                  "jump kernel launch");
+    }
+    for (size_t k = 0; part == 1 && k < launches.size(); k++) {
         const Launch &L = launches[k];
-        hip_check(launch_items(L.bits, L.cls, d_items + L.first, L.count, ctx->d_params, d_queue + 2 * k, L.grid, s),
+        hip_check(launch_items(L.bits, L.cls, L.layout, d_items + L.first, L.count, ctx->d_params, d_queue + 2 * k,
+                               L.grid, s),
                   "kernel launch");
     }
 }
 
+void Plan::configure(int layout, int bpc) {
+    for (auto &L : launches) {
+        if (layout >= 0) {
+            if (layout >= kLayouts)
+                throw Error("unknown layout");
+            L.layout = layout;
+        }
+        int cap = ctx->blocks_per_cu[L.bits == 64][L.cls][L.layout];
+        if (bpc > 0)
+            cap = std::min(cap, bpc);
+        L.grid = std::min(L.count, ctx->cus * cap);
+    }
+}
+
 void Plan::free_device() {
+    if (d_jitems)
+        (void)hipFree(d_jitems);
+    if (d_masks)
+        (void)hipFree(d_masks);
+    if (d_jstates)
+        (void)hipFree(d_jstates);
+    d_jitems = nullptr;
+    d_masks = nullptr;
+    d_jstates = nullptr;
     if (d_items)
         (void)hipFree(d_items);
     if (d_polys)
@@ -540,41 +720,47 @@ struct mtgp_plan {
     Plan plan;
 };
 
-// Applies a jump polynomial (hex) to device window `in`, result in `out`.
+// Applies a jump polynomial (hex) to device window `in`, result in `out`,
+// with the batched jump kernel (one jump).
 static void device_jump(Context &c, int slot, int bits, int mexp, const void *in, void *out, const char *hex) {
     gf2::Poly P;
     if (!gf2::from_hex(hex, P))
         throw Error("jump string contains a non-hex character");
-    const int pw = std::max<int>(1, (int)((strlen(hex) * 4 + 31) / 32));
-    std::vector<uint32_t> words = gf2::to_words32(P, pw);
-    uint32_t *d_poly = nullptr;
-    Item *d_item = nullptr;
+    // Applied literally, as mtgp32_fast_jump does: every coefficient of the
+    // string, whatever its degree (the kernel runs the sequence that far).
+    const std::vector<uint32_t> words = gf2::to_words32(P, std::max<int>(1, (int)((strlen(hex) * 4 + 31) / 32)));
+    std::vector<uint64_t> masks;
+    JumpItem ji{};
+    ji.nchunks = append_jump_masks({&words}, masks);
+    ji.state_in = (uint64_t)(uintptr_t)in;
+    ji.states_out = (uint64_t)(uintptr_t)out;
+    ji.nb = 1;
+    ji.param = slot;
+    ji.state_bytes = state_words(bits, mexp) * (bits / 8);
+    uint64_t *d_masks = nullptr;
+    JumpItem *d_item = nullptr;
     uint32_t *d_q = nullptr;
-    try {
-        hip_check(hipMalloc((void **)&d_poly, words.size() * 4), "hipMalloc");
-        hip_check(hipMemcpy(d_poly, words.data(), words.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
-        Item it{};
-        it.state_in = (uint64_t)(uintptr_t)in;
-        it.state_out = (uint64_t)(uintptr_t)out;
-        it.poly = (uint64_t)(uintptr_t)d_poly;
-        it.poly_words = pw;
-        it.param = slot;
-        hip_check(hipMalloc((void **)&d_item, sizeof(Item)), "hipMalloc");
-        hip_check(hipMemcpy(d_item, &it, sizeof(Item), hipMemcpyHostToDevice), "hipMemcpy");
-        hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
-        hip_check(hipMemset(d_q, 0, 8), "hipMemset");
-        hip_check(launch_items(bits, kernel_class(bits, state_words(bits, mexp)), d_item, 1, c.d_params, d_q, 1, 0),
-                  "kernel launch");
-        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    } catch (...) {
-        (void)hipFree(d_poly);
+    auto cleanup = [&] {
+        (void)hipFree(d_masks);
         (void)hipFree(d_item);
         (void)hipFree(d_q);
+    };
+    try {
+        hip_check(hipMalloc((void **)&d_masks, masks.size() * 8), "hipMalloc");
+        hip_check(hipMemcpy(d_masks, masks.data(), masks.size() * 8, hipMemcpyHostToDevice), "hipMemcpy");
+        ji.masks = (uint64_t)(uintptr_t)d_masks;
+        hip_check(hipMalloc((void **)&d_item, sizeof(JumpItem)), "hipMalloc");
+        hip_check(hipMemcpy(d_item, &ji, sizeof(JumpItem), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
+        hip_check(hipMemset(d_q, 0, 8), "hipMemset");
+        hip_check(launch_jumps(bits, kernel_class(bits, state_words(bits, mexp)), d_item, 1, c.d_params, d_q, 1, 0),
+                  "jump kernel launch");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    } catch (...) {
+        cleanup();
         throw;
     }
-    (void)hipFree(d_poly);
-    (void)hipFree(d_item);
-    (void)hipFree(d_q);
+    cleanup();
 }
 
 template <class Gen, class Word>
@@ -961,6 +1147,22 @@ int mtgp_plan_commit(mtgp_plan_t *p) {
 
 void mtgp_plan_destroy(mtgp_plan_t *p) { delete p; }
 
+int mtgp_plan_launch_part(mtgp_plan_t *p, int part, void *stream) {
+    API_BEGIN
+    if (part != 0 && part != 1)
+        throw Error("part must be 0 (start-state jumps) or 1 (generation)");
+    p->plan.launch_part(part, (hipStream_t)stream);
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_plan_configure(mtgp_plan_t *p, int layout, int blocks_per_cu) {
+    API_BEGIN
+    p->plan.configure(layout, blocks_per_cu);
+    return 0;
+    API_END(-1)
+}
+
 int mtgp_plan_stats(const mtgp_plan_t *p, int *items, int *launches, double *plan_ms) {
     if (items)
         *items = (int)p->plan.items.size();
@@ -1007,7 +1209,7 @@ int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) 
         throw Error("jump string contains a non-hex character");
     Context &c = *s->ctx;
     auto mod = modulus_for(c, s->slot[0]);
-    J = mod->reduce(J);
+    J = mod->reduce_any(J);
     // c_i = J^i mod P, chained in blocks so that host threads share the work.
     const int pw = (s->mexp[0] + 31) / 32;
     std::vector<std::vector<uint32_t>> polys((size_t)G);
@@ -1036,8 +1238,9 @@ int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) 
     // Base window to the device, then one jump item per generator.
     const size_t wbytes = (size_t)s->n[0] * (size_t)(s->bits[0] / 8);
     void *d_base = nullptr;
-    uint32_t *d_polys = nullptr, *d_q = nullptr;
-    Item *d_items = nullptr;
+    uint64_t *d_polys = nullptr;
+    uint32_t *d_q = nullptr;
+    JumpItem *d_items = nullptr;
     auto cleanup = [&] {
         (void)hipFree(d_base);
         (void)hipFree(d_polys);
@@ -1047,30 +1250,38 @@ int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) 
     try {
         hip_check(hipMalloc(&d_base, wbytes), "hipMalloc");
         hip_check(hipMemcpy(d_base, base, wbytes, hipMemcpyHostToDevice), "hipMemcpy");
-        std::vector<uint32_t> flat((size_t)G * (size_t)pw);
-        for (int g = 0; g < G; g++)
-            std::copy(polys[(size_t)g].begin(), polys[(size_t)g].end(), flat.begin() + (size_t)g * (size_t)pw);
-        hip_check(hipMalloc((void **)&d_polys, flat.size() * 4), "hipMalloc");
-        hip_check(hipMemcpy(d_polys, flat.data(), flat.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
-        std::vector<Item> items((size_t)G);
-        for (int g = 0; g < G; g++) {
-            Item &it = items[(size_t)g];
-            it = Item{};
-            it.state_in = (uint64_t)(uintptr_t)d_base;
-            it.state_out = (uint64_t)(uintptr_t)s->state(g, 0);
-            if (g > 0) {
-                it.poly = (uint64_t)(uintptr_t)(d_polys + (size_t)g * (size_t)pw);
-                it.poly_words = pw;
-            }
-            it.param = s->slot[0];
+        // Jump items of kJumpBatch consecutive generators; their slot-0
+        // states are equally spaced in the state buffer.
+        const size_t stride = G > 1 ? (size_t)((char *)s->state(1, 0) - (char *)s->state(0, 0)) : wbytes;
+        std::vector<uint64_t> masks;
+        std::vector<JumpItem> items;
+        for (int g0 = 0; g0 < G; g0 += kJumpBatch) {
+            const int nb = std::min(kJumpBatch, G - g0);
+            std::vector<const std::vector<uint32_t> *> ps;
+            for (int b = 0; b < nb; b++)
+                ps.push_back(&polys[(size_t)(g0 + b)]);
+            JumpItem ji{};
+            ji.state_in = (uint64_t)(uintptr_t)d_base;
+            ji.masks = masks.size() * 8;
+            ji.nchunks = append_jump_masks(ps, masks);
+            ji.states_out = (uint64_t)(uintptr_t)s->state(g0, 0);
+            ji.nb = nb;
+            ji.param = s->slot[0];
+            ji.state_bytes = (int32_t)stride;
+            items.push_back(ji);
         }
-        hip_check(hipMalloc((void **)&d_items, items.size() * sizeof(Item)), "hipMalloc");
-        hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMalloc((void **)&d_polys, masks.size() * 8), "hipMalloc");
+        hip_check(hipMemcpy(d_polys, masks.data(), masks.size() * 8, hipMemcpyHostToDevice), "hipMemcpy");
+        for (auto &ji : items)
+            ji.masks += (uint64_t)(uintptr_t)d_polys;
+        hip_check(hipMalloc((void **)&d_items, items.size() * sizeof(JumpItem)), "hipMalloc");
+        hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(JumpItem), hipMemcpyHostToDevice), "hipMemcpy");
         hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
         hip_check(hipMemset(d_q, 0, 8), "hipMemset");
         const int bits = s->bits[0], cls = kernel_class(bits, s->n[0]);
-        const int grid = std::min(G, c.cus * c.blocks_per_cu[bits == 64][cls]);
-        hip_check(launch_items(bits, cls, d_items, G, c.d_params, d_q, grid, 0), "kernel launch");
+        const int n_items = (int)items.size();
+        const int grid = std::min(n_items, c.cus * c.jump_blocks_per_cu[bits == 64][cls]);
+        hip_check(launch_jumps(bits, cls, d_items, n_items, c.d_params, d_q, grid, 0), "jump kernel launch");
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (...) {
         cleanup();

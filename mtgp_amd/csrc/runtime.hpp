@@ -31,7 +31,13 @@ struct Context {
     std::string arch;
     DevParam *d_params = nullptr;
     int n_params = 0, param_capacity = 0;
-    int blocks_per_cu[2][3] = {{1, 1, 1}, {1, 1, 1}};  // [bits==64][class]
+    // Occupancy limit [bits==64][class][layout] and the default layout.
+    int blocks_per_cu[2][3][2] = {};
+    int jump_blocks_per_cu[2][3] = {};
+    int layout[2][3] = {{1, 1, 1}, {1, 1, 1}};
+    // Resident workgroups per CU used by default (measured best for the
+    // generation kernels: 16 of the 32 the occupancy limit allows).
+    int default_bpc = 16;
     struct Custom {
         DevParam rec;
         unsigned char sha1[20];
@@ -55,6 +61,10 @@ DevParam dev_param_of(const mtgp64_params_fast_t &p);
 std::shared_ptr<gf2::Modulus> modulus_for(const DevParam &d, const unsigned char *sha1);
 std::shared_ptr<gf2::Modulus> modulus_for(Context &c, int slot);
 
+// Per-residue coefficient masks of up to kJumpBatch jump polynomials
+// (layout in common.hpp), appended to `out`; returns the chunk count.
+int append_jump_masks(const std::vector<const std::vector<uint32_t> *> &polys, std::vector<uint64_t> &out);
+
 // A generator as the planner sees it.
 struct GenRef {
     int bits = 32, mexp = 0, n = 0, slot = 0;
@@ -62,16 +72,25 @@ struct GenRef {
     void *state_out = nullptr;  // where the window after the plan goes
 };
 
-uint64_t auto_segment(const Context &c, uint64_t total_numbers);
+uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, int slots);
 
 // A list of device Items plus the launches that run them.
 struct Plan {
     struct Launch {
-        int bits, cls, first, count, grid;
+        int bits, cls, layout, first, count, grid;
     };
     Context *ctx = nullptr;
+    // Start states of segments: 0 = jump inside the generation kernel
+    // (Item.poly), 1 = a batched jump launch first (JumpItem) writing them
+    // to a state arena, so the generation launch is jump-free.
+    int jump_mode = 1;
     std::vector<Item> items;
     std::vector<Launch> launches;
+    std::vector<JumpItem> jitems;
+    std::vector<Launch> jlaunches;  // layout unused
+    JumpItem *d_jitems = nullptr;
+    uint64_t *d_masks = nullptr;
+    char *d_jstates = nullptr;
     std::vector<int> commits;  // generators whose state moves
     Item *d_items = nullptr;
     uint32_t *d_polys = nullptr;
@@ -82,6 +101,10 @@ struct Plan {
     void build(Context &c, const std::vector<GenRef> &gens, const std::vector<mtgp_request_t> &reqs,
                uint64_t segment);
     void launch(hipStream_t s);
+    void launch_part(int part, hipStream_t s);  // 0 jumps, 1 generation
+    // Overrides the step layout and/or the resident workgroups per CU of
+    // every launch (-1 keeps the current value).
+    void configure(int layout, int blocks_per_cu);
     void free_device();
     Plan() = default;
     Plan(const Plan &) = delete;

@@ -1,0 +1,28 @@
+// Entry point of the kernel emulation library (tests/emu): runs a list of
+// Items through the library's own kernels.hip compiled as host C++.
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+extern "C" int emu_run_items(int bits, int cls, int layout, const mtgp::Item *items, int n_items, int grid) {
+    static std::vector<mtgp::DevParam> params;
+    if (params.empty())
+        for (int k = 0; k < mtgp::kNumSets; k++)
+            params.push_back(mtgp::host_param(k));
+    uint32_t queue[2] = {0, 0};
+    if (mtgp::launch_items(bits, cls, layout, items, n_items, params.data(), queue, grid, nullptr) != hipSuccess)
+        return -1;
+    return (queue[0] == 0 && queue[1] == 0) ? 0 : -2;  // the counters must reset themselves
+}
+
+extern "C" int emu_run_jumps(int bits, int cls, const mtgp::JumpItem *items, int n_items, int grid) {
+    static std::vector<mtgp::DevParam> params;
+    if (params.empty())
+        for (int k = 0; k < mtgp::kNumSets; k++)
+            params.push_back(mtgp::host_param(k));
+    uint32_t queue[2] = {0, 0};
+    if (mtgp::launch_jumps(bits, cls, items, n_items, params.data(), queue, grid, nullptr) != hipSuccess)
+        return -1;
+    return (queue[0] == 0 && queue[1] == 0) ? 0 : -2;
+}

@@ -1,0 +1,64 @@
+"""ctypes front end of the kernel emulation library (test infrastructure).
+
+Items point at host numpy arrays, which the emulated kernels read and write
+like device memory."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ITEM = np.dtype([("out", "<u8"), ("count", "<u8"), ("state_in", "<u8"), ("state_out", "<u8"),
+                 ("poly", "<u8"), ("poly_words", "<i4"), ("param", "<i4"), ("kind", "<i4"),
+                 ("pad", "<i4"), ("pad2", "<u8")])
+assert ITEM.itemsize == 64
+
+
+def load():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+    lib = C.CDLL(os.path.join(HERE, "libmtgp_emu.so"), mode=os.RTLD_LOCAL)
+    lib.emu_run_items.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
+    lib.emu_run_jumps.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
+    return lib
+
+
+JUMP_ITEM = np.dtype([("state_in", "<u8"), ("masks", "<u8"), ("states_out", "<u8"), ("nb", "<i4"),
+                      ("nchunks", "<i4"), ("param", "<i4"), ("state_bytes", "<i4"), ("pad", "<u8", (3,))])
+assert JUMP_ITEM.itemsize == 64
+BATCH, CHUNK = 4, 1024
+
+
+def jump_masks(coeff_lists):
+    """Per-residue masks of up to BATCH jumps (lists of set coefficient
+    indices): word c*64+rho holds bit 16b+q for coefficient c*1024+64q+rho."""
+    top = max([max(c) for c in coeff_lists if len(c)] + [0])
+    nchunks = top // CHUNK + 1
+    m = np.zeros(nchunks * 64, np.uint64)
+    for b, cs in enumerate(coeff_lists):
+        for k in cs:
+            c, r = divmod(int(k), CHUNK)
+            q, rho = divmod(r, 64)
+            m[c * 64 + rho] |= np.uint64(1 << (16 * b + q))
+    return m, nchunks
+
+
+def set_id(bits, mexp, index):
+    t = {(32, 11213): 0, (32, 23209): 1, (32, 44497): 2, (64, 23209): 3, (64, 44497): 4, (64, 110503): 5}
+    return t[(bits, mexp)] * 128 + index
+
+
+def kernel_class(bits, n):
+    lim = (351, 726, 1391) if bits == 32 else (363, 696, 1727)
+    return next(i for i, v in enumerate(lim) if n <= v)
+
+
+def run(lib, bits, items, layout, grid=None):
+    arr = np.zeros(len(items), ITEM)
+    n_max = 0
+    for k, it in enumerate(items):
+        for f, v in it.items():
+            arr[k][f] = v
+    cls = max(kernel_class(bits, it["n"]) for it in items) if items else 0
+    rc = lib.emu_run_items(bits, cls, layout, arr.ctypes.data, len(items), grid or len(items))
+    assert rc == 0, rc

@@ -117,9 +117,10 @@ def test_jump_equals_stepping(mtgp, oracle, bits, mexp):
         assert_same(a, b, "s=%d" % s)
 
 
-def test_genset_segmented_requests(mtgp, oracle):
+@pytest.mark.parametrize("layout", [0, 1])
+def test_genset_segmented_requests(mtgp, oracle, layout):
     """Several generators, requests with skips, split into short segments so
-    that most segments start from a device jump."""
+    that most segments start from a device jump; both step layouts."""
     specs = [(32, 11213, 3), (32, 23209, 9), (64, 23209, 4), (32, 11213, 100), (64, 44497, 2)]
     gs = mtgp.GenSet(specs)
     seeds = [11, 22, 33, 44, 55]
@@ -129,9 +130,9 @@ def test_genset_segmented_requests(mtgp, oracle):
     for g, (bits, mexp, idx) in enumerate(specs):
         p = oracle.params(bits, mexp, idx)
         windows[g] = (p, oracle.init_state(p, seeds[g]))
-    layout = [(0, 0, 70000, 0), (0, 2, 30001, 1234), (1, 1, 50000, 7), (2, 0, 40000, 0),
-              (3, 4, 123457, 99999), (2, 3, 1, 3), (4, 2, 20000, 0), (4, 0, 5, 5)]
-    for g, kind, count, skip in layout:
+    requests = [(0, 0, 70000, 0), (0, 2, 30001, 1234), (1, 1, 50000, 7), (2, 0, 40000, 0),
+                (3, 4, 123457, 99999), (2, 3, 1, 3), (4, 2, 20000, 0), (4, 0, 5, 5)]
+    for g, kind, count, skip in requests:
         bits = specs[g][0]
         dt = mtgp.out_dtype(bits, kind)
         buf = mtgp.DeviceBuffer(count * np.dtype(dt).itemsize)
@@ -145,6 +146,7 @@ def test_genset_segmented_requests(mtgp, oracle):
         expect.append(want)
     plan = gs.plan(plan_reqs, segment=8192)
     assert plan.stats()["items"] > 40
+    plan.configure(layout=layout)
     plan.launch()
     mtgp.synchronize()
     plan.commit()
@@ -152,3 +154,39 @@ def test_genset_segmented_requests(mtgp, oracle):
         assert_same(buf.to_host(dt, count), expect[k], "request %d" % k)
     for g in range(len(specs)):
         assert_same(gs.state(g), windows[g][1], "state %d" % g)
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("bits,mexp", [(32, 11213), (64, 23209)])
+def test_no_write_past_requested_outputs(mtgp, oracle, layout, bits, mexp):
+    """Every request of a plan writes exactly its own range: guard words
+    around each output stay untouched (tails of every length mod 256 and
+    odd output offsets)."""
+    eb = bits // 8
+    counts = [1, 2, 3, 5, 63, 64, 65, 255, 256, 257, 511, 1000, 4099]
+    gs = mtgp.GenSet([(bits, mexp, 3)] * len(counts))
+    gs.init_seed(list(range(len(counts))))
+    guard = 64
+    layout_words = [(guard + c + guard) for c in counts]
+    buf = mtgp.DeviceBuffer(sum(layout_words) * eb + 64)
+    pattern = np.full(sum(layout_words) * eb + 64, 0xA5, np.uint8)
+    buf.from_host(pattern)
+    reqs, offs, off = [], [], 0
+    for g, c in enumerate(counts):
+        start = off + guard * eb + (eb if g % 2 else 0)
+        reqs.append((g, g % 5, c, 0, buf.address(start)))
+        offs.append(start)
+        off += layout_words[g] * eb
+    plan = gs.plan(reqs, segment=1 << 20)
+    plan.configure(layout=layout)
+    plan.launch()
+    mtgp.synchronize()
+    host = buf.to_host(np.uint8)
+    mask = np.ones(host.size, bool)
+    for g, c in enumerate(counts):
+        mask[offs[g]:offs[g] + c * eb] = False
+        p = oracle.params(bits, mexp, 3)
+        want, _ = oracle.generate(p, oracle.init_state(p, g), g % 5, c)
+        got = host[offs[g]:offs[g] + c * eb].view(want.dtype)
+        assert_same(got, want, "request %d" % g)
+    assert (host[mask] == 0xA5).all(), "bytes outside the requested ranges were written"

@@ -1,0 +1,137 @@
+"""The library's kernels.hip, compiled as host C++ and run on host threads
+(tests/emu), checked bit-exactly against the oracle.  This exercises the
+kernel logic -- step layouts, heads and tails, LDS relocation, the device
+jump, the self-resetting work queue -- without a GPU."""
+import numpy as np
+import pytest
+
+import pyemu
+
+SETS = [(32, 11213), (32, 23209), (32, 44497), (64, 23209), (64, 44497), (64, 110503)]
+
+
+@pytest.fixture(scope="module")
+def emu():
+    return pyemu.load()
+
+
+def _bits(a):
+    return a.view(np.uint32 if a.dtype.itemsize == 4 else np.uint64)
+
+
+def _run(emu, bits, items, layout, grid=None):
+    arr = np.zeros(len(items), pyemu.ITEM)
+    for k, it in enumerate(items):
+        for f, v in it.items():
+            if f != "n":
+                arr[k][f] = v
+    cls = max(pyemu.kernel_class(bits, it["n"]) for it in items)
+    rc = emu.emu_run_items(bits, cls, layout, arr.ctypes.data, len(items), grid or len(items))
+    assert rc == 0, "emulated launch failed (%d)" % rc
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("bits,mexp", SETS)
+def test_items_all_kinds(emu, oracle, bits, mexp, layout):
+    """One launch, one item per kind with different counts (covering the
+    head, whole steps, LDS relocation and the tail), unaligned outputs,
+    fewer workgroups than items so that the queue hands out work."""
+    p = oracle.params(bits, mexp, 5)
+    wd = np.uint32 if bits == 32 else np.uint64
+    eb = bits // 8
+    w0 = oracle.init_state(p, 2024)
+    counts = [(0, 1), (1, 259), (2, 1030), (3, 0), (4, 777), (0, 2049)]
+    items, keep, want = [], [], []
+    for k, (kind, n) in enumerate(counts):
+        raw = np.zeros((n + 16) * eb + 16, np.uint8)
+        off = (4 * k) % 16 if bits == 32 else (8 * k) % 16
+        sin, sout = w0.copy(), np.zeros_like(w0)
+        base = raw.ctypes.data + (-raw.ctypes.data) % 16 + off
+        items.append(dict(out=base, count=n, state_in=sin.ctypes.data, state_out=sout.ctypes.data,
+                          param=pyemu.set_id(bits, mexp, 5), kind=kind, n=p.n))
+        keep.append((raw, base, sin, sout, n))
+        e, w = oracle.generate(p, w0, kind, n)
+        want.append((e, w))
+    _run(emu, bits, items, layout, grid=2)
+    for (raw, base, sin, sout, n), (e, w) in zip(keep, want):
+        start = base - raw.ctypes.data
+        got = raw[start:start + n * eb].view(e.dtype)
+        assert (_bits(got) == _bits(e)).all()
+        assert (raw[start + n * eb:] == 0).all(), "write past the requested outputs"
+        assert (sout == w).all()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("bits,mexp", SETS[:4])
+def test_item_jump(emu, oracle, bits, mexp, layout):
+    """Item with a jump polynomial: the window is replaced by its jump
+    before generating; compared with the oracle's jump."""
+    p = oracle.params(bits, mexp, 9)
+    w0 = oracle.init_state(p, 31337)
+    words = (mexp + 31) // 32
+    rng = np.random.default_rng(mexp)
+    poly = rng.integers(0, 2**32, words, dtype=np.uint64).astype(np.uint32)
+    poly[-1] &= (1 << (mexp % 32)) - 1 if mexp % 32 else 0xFFFFFFFF
+    # hex nibbles low first (polytostring format), from the bit vector
+    bitsv = np.unpackbits(poly.view(np.uint8), bitorder="little")
+    nib = [int(bitsv[4 * i]) | int(bitsv[4 * i + 1]) << 1 | int(bitsv[4 * i + 2]) << 2 | int(bitsv[4 * i + 3]) << 3
+           for i in range(len(bitsv) // 4)]
+    hexs = "".join("0123456789abcdef"[v] for v in nib).rstrip("0")
+    wj = oracle.jump(p, w0, hexs)
+    n = 600
+    eb = bits // 8
+    out = np.zeros(n + 8, np.uint32 if bits == 32 else np.uint64)
+    sin, sout = w0.copy(), np.zeros_like(w0)
+    _run(emu, bits, [dict(out=out.ctypes.data, count=n, state_in=sin.ctypes.data, state_out=sout.ctypes.data,
+                          poly=poly.ctypes.data, poly_words=words, param=pyemu.set_id(bits, mexp, 9), kind=0,
+                          n=p.n)], layout)
+    e, w = oracle.generate(p, wj, 0, n)
+    assert (out[:n] == e).all()
+    assert (out[n:] == 0).all()
+    assert (sout == w).all()
+
+
+def _hex_of(coeffs, top):
+    bits = np.zeros(top + 1, np.uint8)
+    bits[list(coeffs)] = 1
+    bits = np.concatenate([bits, np.zeros((-len(bits)) % 4, np.uint8)])
+    nib = bits[0::4] | bits[1::4] << 1 | bits[2::4] << 2 | bits[3::4] << 3
+    return "".join("0123456789abcdef"[v] for v in nib).rstrip("0")
+
+
+@pytest.mark.parametrize("bits,mexp", SETS[:5])
+def test_batched_jumps(emu, oracle, bits, mexp):
+    """The batched jump kernel: several items of 1..4 jumps from one base
+    window, each compared with the oracle's literal jump -- including a
+    polynomial of degree above mexp (the reference applies every
+    coefficient of the string)."""
+    p = oracle.params(bits, mexp, 11)
+    wd = np.uint32 if bits == 32 else np.uint64
+    base = oracle.init_state(p, 99)
+    rng = np.random.default_rng(mexp + bits)
+    jobs = []
+    for nb in (1, 3, 4):
+        cl = []
+        for b in range(nb):
+            top = mexp - 1 if b != 2 else mexp + 700
+            cl.append(sorted(set(rng.integers(0, top + 1, size=top // 2).tolist()) | {top}))
+        jobs.append(cl)
+    arr = np.zeros(len(jobs), pyemu.JUMP_ITEM)
+    keep = []
+    for k, cl in enumerate(jobs):
+        masks, nchunks = pyemu.jump_masks(cl)
+        outs = np.zeros((len(cl), p.n), wd)
+        keep.append((masks, outs))
+        arr[k]["state_in"] = base.ctypes.data
+        arr[k]["masks"] = masks.ctypes.data
+        arr[k]["states_out"] = outs.ctypes.data
+        arr[k]["nb"] = len(cl)
+        arr[k]["nchunks"] = nchunks
+        arr[k]["param"] = pyemu.set_id(bits, mexp, 11)
+        arr[k]["state_bytes"] = p.n * (bits // 8)
+    rc = emu.emu_run_jumps(bits, pyemu.kernel_class(bits, p.n), arr.ctypes.data, len(jobs), 2)
+    assert rc == 0
+    for (masks, outs), cl in zip(keep, jobs):
+        for b, cs in enumerate(cl):
+            want = oracle.jump(p, base, _hex_of(cs, cs[-1]))
+            assert (outs[b] == want).all(), "jump %d of %d" % (b, len(cl))
