@@ -25,8 +25,9 @@ struct alignas(16) Item {
     int32_t poly_words;  // number of uint32_t coefficient words
     int32_t param;       // index into the device parameter table
     int32_t kind;        // 0 int, 1 [1,2), 2 [0,1), 3 (0,1], 4 (0,1)
-    int32_t pad;
-    uint64_t pad2;
+    int32_t parts;       // >1: the start window is the XOR of `parts`
+                         // partial windows, part_bytes apart, at state_in
+    uint64_t part_bytes;
 };
 static_assert(sizeof(Item) == 64, "Item layout");
 
@@ -37,15 +38,19 @@ static_assert(sizeof(Item) == 64, "Item layout");
 constexpr int kJumpBatch = 4;
 constexpr int kJumpSteps = 4;                  // 256-word steps per chunk
 constexpr int kJumpChunk = kJumpSteps * 256;   // positions per chunk
+// Chunks [c0, c1) are folded; the sequence is generated from chunk 0.  A
+// jump split into parts writes partial windows whose XOR is the jumped
+// window (Item.parts).
 struct alignas(16) JumpItem {
     uint64_t state_in;
-    uint64_t masks;       // const uint16_t *
+    uint64_t masks;       // const uint64_t *: [chunk][64 residues]
     uint64_t states_out;
     int32_t nb;
-    int32_t nchunks;
+    int32_t c0, c1;
     int32_t param;
     int32_t state_bytes;
-    uint64_t pad[3];
+    int32_t pad0;
+    uint64_t pad[2];
 };
 static_assert(sizeof(JumpItem) == 64, "JumpItem layout");
 

@@ -258,9 +258,10 @@ template <int KIND, int NJ, int NBUF, int LAYOUT>
 __device__ void generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int o, int h,
                            gu32 *__restrict__ out, uint64_t count, gu32 *__restrict__ state_out) {
     const int l = threadIdx.x;
-    // Head: h < 4 words, one per lane, so that the main loop stores are
-    // 16-byte aligned; the window base becomes o + h == 0 mod 4.
-    if (h > 0) {
+    // Head (layout 0): h < 4 words, one per lane, so that the main loop
+    // stores are 16-byte aligned; the window base becomes o + h == 0 mod 4.
+    // (Layout 1 takes its head below as a partial step.)
+    if (LAYOUT == 0 && h > 0) {
         if (l < h) {
             const int q = o + l;
             const uint32_t r = rec32(p, tab, buf[q], buf[q + 1], buf[q + p.pos]);
@@ -271,9 +272,21 @@ __device__ void generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int
     }
     // count - h numbers: whole steps, then one partial step.  (Written
     // without a select on the 64-bit remainder.)
-    int b = o + h;
-    gu32 *dst = out + h;
-    const uint64_t rest = count - (uint64_t)h;
+    int b = o + (LAYOUT == 0 ? h : 0);
+    gu32 *dst = out;
+    uint64_t rest = count;
+    if (LAYOUT == 0) {
+        dst += h;
+        rest -= (uint64_t)h;
+    } else if (h > 0) {
+        // Layout 1: h < 32 numbers so that every later 256-byte store starts
+        // on a 128-byte line (no partially written lines).
+        step32s<KIND, true>(buf, tab, p, b, dst, (uint32_t)h);
+        wave_sync();
+        b += h;
+        dst += h;
+        rest -= (uint64_t)h;
+    }
     for (uint64_t s = rest / kStep; s > 0; s--) {
         if (b + p.n + kStep > NBUF) {
             relocate32<NJ>(buf, b, p.n);
@@ -330,10 +343,13 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
         gu32 *out = reinterpret_cast<gu32 *>(I.out);
         gu32 *state_out = reinterpret_cast<gu32 *>(I.state_out);
         const gu32 *state_in = reinterpret_cast<const gu32 *>(I.state_in);
-        uint32_t h = LAYOUT == 0 ? (uint32_t)((16u - (uint32_t)(I.out & 15)) & 15) >> 2 : 0u;
+        // Numbers before the first aligned store: to 16 bytes (layout 0) or
+        // to a 128-byte line (layout 1).
+        uint32_t h = LAYOUT == 0 ? (uint32_t)((16u - (uint32_t)(I.out & 15)) & 15) >> 2
+                                 : (uint32_t)((128u - (uint32_t)(I.out & 127)) & 127) >> 2;
         if ((uint64_t)h > count)
             h = (uint32_t)count;
-        const int o = (int)((4u - h) & 3u);
+        const int o = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
         wave_sync();
         if (I.poly) {
             uint32_t acc[NJ];
@@ -344,8 +360,14 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
                 if (l + kWave * m < p.n)
                     buf[o + l + kWave * m] = acc[m];
         } else {
-            for (int j = l; j < p.n; j += kWave)
-                buf[o + j] = state_in[j];
+            const int parts = I.parts > 1 ? I.parts : 1;
+            const uint64_t pstride = I.part_bytes / 4;
+            for (int j = l; j < p.n; j += kWave) {
+                uint32_t v = state_in[j];
+                for (int q = 1; q < parts; q++)
+                    v ^= state_in[(uint64_t)q * pstride + (uint64_t)j];
+                buf[o + j] = v;
+            }
         }
         wave_sync();
         switch (I.kind) {
@@ -602,7 +624,7 @@ template <int KIND, int NJ, int NBUF, int LAYOUT>
 __device__ void generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int o, int h,
                            gu32 *__restrict__ out, uint64_t count, gu64 *__restrict__ state_out) {
     const int l = threadIdx.x;
-    if (h > 0) {
+    if (LAYOUT == 0 && h > 0) {
         if (l < h) {
             const int q = o + l;
             uint32_t rh, rl, oh, ol;
@@ -614,9 +636,19 @@ __device__ void generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, cons
         }
         wave_sync();
     }
-    int b = o + h;
-    gu32 *dst = out + 2 * h;
-    const uint64_t rest = count - (uint64_t)h;
+    int b = o + (LAYOUT == 0 ? h : 0);
+    gu32 *dst = out;
+    uint64_t rest = count;
+    if (LAYOUT == 0) {
+        dst += 2 * h;
+        rest -= (uint64_t)h;
+    } else if (h > 0) {
+        step64s<KIND, true>(bh, bl, tab, p, b, dst, (uint32_t)h);
+        wave_sync();
+        b += h;
+        dst += 2 * h;
+        rest -= (uint64_t)h;
+    }
     for (uint64_t s = rest / kStep; s > 0; s--) {
         if (b + p.n + kStep > NBUF) {
             relocate64<NJ>(bh, bl, b, p.n);
@@ -675,10 +707,11 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         gu32 *out = reinterpret_cast<gu32 *>(I.out);
         gu64 *state_out = reinterpret_cast<gu64 *>(I.state_out);
         const gu64 *state_in = reinterpret_cast<const gu64 *>(I.state_in);
-        uint32_t h = (LAYOUT == 0 && (I.out & 15)) ? 1u : 0u;  // 8-byte outputs
+        uint32_t h = LAYOUT == 0 ? ((I.out & 15) ? 1u : 0u)  // 8-byte outputs
+                                 : (uint32_t)((128u - (uint32_t)(I.out & 127)) & 127) >> 3;
         if ((uint64_t)h > count)
             h = (uint32_t)count;
-        const int o = (int)((4u - h) & 3u);
+        const int o = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
         wave_sync();
         if (I.poly) {
             uint32_t ah[NJ], al[NJ];
@@ -691,8 +724,12 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
                     bl[o + l + kWave * m] = al[m];
                 }
         } else {
+            const int parts = I.parts > 1 ? I.parts : 1;
+            const uint64_t pstride = I.part_bytes / 8;
             for (int j = l; j < p.n; j += kWave) {
-                const uint64_t v = state_in[j];
+                uint64_t v = state_in[j];
+                for (int q = 1; q < parts; q++)
+                    v ^= state_in[(uint64_t)q * pstride + (uint64_t)j];
                 bh[o + j] = (uint32_t)(v >> 32);
                 bl[o + j] = (uint32_t)v;
             }
@@ -734,6 +771,13 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 
 typedef GMEM uint64_t gu64c;
 
+// Lane `src`'s 64-bit value as a wave-uniform value.
+__device__ __forceinline__ uint64_t lane_mask64(uint64_t v, int src) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, src);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 template <int NMAX>
 __global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restrict__ items, int n_items,
                                                          const DevParam *__restrict__ params,
@@ -771,13 +815,15 @@ __global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restr
 #pragma unroll
             for (int m = 0; m < NJ; m++)
                 acc[b][m] = 0;
-        for (int c = 0; c < I.nchunks; c++) {
+        for (int c = 0; c < I.c1; c++) {
             for (int s = 0; s < K; s++) {
                 step32s<0, false>(buf, tab, p, s * kStep, nullptr, 0);
                 wave_sync();
             }
-            for (int rho = 0; rho < kWave; rho++) {
-                const uint64_t mm = uniform64(mk[c * kWave + rho]);
+            // Lane rho holds residue rho's masks; readlane hands them out.
+            const uint64_t lane_masks = c >= I.c0 ? mk[(uint64_t)c * kWave + l] : 0ull;
+            for (int rho = 0; c >= I.c0 && rho < kWave; rho++) {
+                const uint64_t mm = lane_mask64(lane_masks, rho);
                 if (!mm)
                     continue;
                 uint32_t R[QMAX + NJ];
@@ -867,13 +913,15 @@ __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restr
 #pragma unroll
             for (int m = 0; m < NJ; m++)
                 ah[b][m] = al[b][m] = 0;
-        for (int c = 0; c < I.nchunks; c++) {
+        for (int c = 0; c < I.c1; c++) {
             for (int s = 0; s < K; s++) {
                 step64s<0, false>(bh, bl, tab, p, s * kStep, nullptr, 0);
                 wave_sync();
             }
-            for (int rho = 0; rho < kWave; rho++) {
-                const uint64_t mm = uniform64(mk[c * kWave + rho]);
+            // Lane rho holds residue rho's masks; readlane hands them out.
+            const uint64_t lane_masks = c >= I.c0 ? mk[(uint64_t)c * kWave + l] : 0ull;
+            for (int rho = 0; c >= I.c0 && rho < kWave; rho++) {
+                const uint64_t mm = lane_mask64(lane_masks, rho);
                 if (!mm)
                     continue;
                 uint32_t RH[QMAX + NJ], RL[QMAX + NJ];
@@ -941,8 +989,8 @@ struct KernelOf<64, NMAX, LAYOUT> {
 
 template <class KO>
 hipError_t launch_one(const Item *items, int n_items, const DevParam *params, uint32_t *queue, int grid,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(KO::fn, dim3(grid), dim3(kWave), 0, s, items, n_items, params, queue);
+                      size_t dyn_lds, hipStream_t s) {
+    hipLaunchKernelGGL(KO::fn, dim3(grid), dim3(kWave), dyn_lds, s, items, n_items, params, queue);
     return hipGetLastError();
 }
 
@@ -974,8 +1022,9 @@ const void *kernel_fn(int bits, int cls, int layout) {
 }
 
 hipError_t launch_items(int bits, int cls, int layout, const Item *items, int n_items, const DevParam *params,
-                        uint32_t *queue, int grid, hipStream_t s) {
-    return with_kernel(bits, cls, layout, [&]<class KO>() { return launch_one<KO>(items, n_items, params, queue, grid, s); });
+                        uint32_t *queue, int grid, size_t dyn_lds, hipStream_t s) {
+    return with_kernel(bits, cls, layout,
+                       [&]<class KO>() { return launch_one<KO>(items, n_items, params, queue, grid, dyn_lds, s); });
 }
 
 namespace {
@@ -992,9 +1041,9 @@ const void *jump_kernel_fn(int bits, int cls) {
 }
 
 hipError_t launch_jumps(int bits, int cls, const JumpItem *items, int n_items, const DevParam *params,
-                        uint32_t *queue, int grid, hipStream_t s) {
+                        uint32_t *queue, int grid, size_t dyn_lds, hipStream_t s) {
     return with_jump_kernel(bits, cls, [&](auto fn) {
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(kWave), 0, s, items, n_items, params, queue);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(kWave), dyn_lds, s, items, n_items, params, queue);
         return hipGetLastError();
     });
 }

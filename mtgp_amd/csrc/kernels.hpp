@@ -16,12 +16,14 @@ const void *kernel_fn(int bits, int cls, int layout);
 // Runs items[0, n_items) (device array, all of word size `bits` and class
 // <= cls) with `grid` persistent workgroups.  `queue` is a device pair of
 // uint32_t counters that is zero on entry and zero again on exit.
+// `dyn_lds` bytes of unused dynamic LDS per workgroup cap how many
+// workgroups a CU holds at once (even placement of one-item-per-wave grids).
 hipError_t launch_items(int bits, int cls, int layout, const Item *items, int n_items, const DevParam *params,
-                        uint32_t *queue, int grid, hipStream_t s);
+                        uint32_t *queue, int grid, size_t dyn_lds, hipStream_t s);
 
 // Batched jump kernel (JumpItem, common.hpp).
 const void *jump_kernel_fn(int bits, int cls);
 hipError_t launch_jumps(int bits, int cls, const JumpItem *items, int n_items, const DevParam *params,
-                        uint32_t *queue, int grid, hipStream_t s);
+                        uint32_t *queue, int grid, size_t dyn_lds, hipStream_t s);
 
 }  // namespace mtgp

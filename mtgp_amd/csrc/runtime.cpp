@@ -114,6 +114,42 @@ void select_device(int dev) {
     g_device = dev;
 }
 
+size_t Context::lds_reserve(const void *fn, int per_cu) {
+    // Off unless asked for: measured neutral at the default 16 per CU.
+    const char *e = getenv("MTGP_LDS_CAP");
+    if (!e || atoi(e) == 0)
+        return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(fn, per_cu);
+    auto it = reserve_cache.find(key);
+    if (it != reserve_cache.end())
+        return it->second;
+    int base = 0;
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, fn, 64, 0), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    size_t dyn = 0;
+    if (base > per_cu) {
+        hipFuncAttributes attr;
+        hip_check(hipFuncGetAttributes(&attr, fn), "hipFuncGetAttributes");
+        int lds_cu = 0;
+        hip_check(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device),
+                  "hipDeviceGetAttribute");
+        // Largest reservation that still allows per_cu workgroups.
+        size_t want = (size_t)lds_cu / (size_t)per_cu;
+        dyn = want > attr.sharedSizeBytes ? want - attr.sharedSizeBytes : 0;
+        dyn &= ~(size_t)255;
+        for (int tries = 0; tries < 64; tries++) {
+            int nb = 0;
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, dyn),
+                      "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+            if (nb <= per_cu || dyn == 0)
+                break;
+            dyn += 256;
+        }
+    }
+    reserve_cache[key] = dyn;
+    return dyn;
+}
+
 // Parameter slot of an arbitrary (host) parameter record: a table set when
 // it equals one, else a custom slot uploaded on first use.
 int Context::slot_of(const DevParam &rec, const unsigned char *sha1) {
@@ -461,38 +497,70 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     std::vector<uint32_t> poly_host;
     std::vector<uint64_t> mask_host;
     std::vector<int> jgroup;
-    std::vector<size_t> arena_off;  // per piece with a jump (mode 1)
+    std::vector<uint64_t> arena_slot(0);  // per list item: arena offset + 1 (0 = none)
     size_t arena_bytes = 0;
     jitems.clear();
-    for (int g = 0; g < G; g++) {
-        const GenRef &gr = gens[(size_t)g];
-        const size_t sbytes = (size_t)gr.n * (size_t)(gr.bits / 8);
-        std::vector<size_t> slot_of_piece(pieces[(size_t)g].size(), (size_t)-1);
-        if (jump_mode == 1) {
+    // Jump groups first (to size the parts), then items.
+    struct Group {
+        int g;
+        std::vector<size_t> pieces;
+        size_t mask_off;  // byte offset into the mask arena
+        int nchunks;
+    };
+    std::vector<Group> groups;
+    if (jump_mode == 1) {
+        for (int g = 0; g < G; g++) {
             std::vector<size_t> need;
             for (size_t k = 0; k < pieces[(size_t)g].size(); k++)
                 if (pieces[(size_t)g][k].off > 0)
                     need.push_back(k);
             for (size_t i0 = 0; i0 < need.size(); i0 += kJumpBatch) {
-                const size_t nb = std::min<size_t>(kJumpBatch, need.size() - i0);
+                Group gr;
+                gr.g = g;
+                gr.pieces.assign(need.begin() + (long)i0, need.begin() + (long)std::min(need.size(), i0 + kJumpBatch));
                 std::vector<const std::vector<uint32_t> *> ps;
-                for (size_t i = 0; i < nb; i++) {
-                    ps.push_back(&polys[(size_t)g][need[i0 + i]]);
-                    slot_of_piece[need[i0 + i]] = arena_bytes + i * sbytes;
-                }
-                JumpItem ji{};
-                ji.state_in = (uint64_t)(uintptr_t)gr.state_in;
-                ji.masks = mask_host.size() * 8;  // byte offset, fixed up below
-                ji.nchunks = append_jump_masks(ps, mask_host);
-                ji.states_out = arena_bytes;      // byte offset, fixed up below
-                ji.nb = (int32_t)nb;
-                ji.param = gr.slot;
-                ji.state_bytes = (int32_t)sbytes;
-                jitems.push_back(ji);
-                jgroup.push_back((gr.bits == 64 ? 3 : 0) + kernel_class(gr.bits, gr.n));
-                arena_bytes += ((nb * sbytes + 255) & ~(size_t)255);
+                for (size_t k : gr.pieces)
+                    ps.push_back(&polys[(size_t)g][k]);
+                gr.mask_off = mask_host.size() * 8;
+                gr.nchunks = append_jump_masks(ps, mask_host);
+                groups.push_back(gr);
             }
         }
+    }
+    // Split each group's chunks over `parts` waves so the jump launch has
+    // about four waves per SIMD; the partial windows are XORed on load.
+    int parts = 1;
+    if (!groups.empty()) {
+        const int target = c.cus * 16;
+        parts = std::max(1, std::min<int>((target + (int)groups.size() - 1) / (int)groups.size(), 8));
+    }
+    std::map<std::pair<int, size_t>, std::pair<uint64_t, int>> slot_of;  // (g, piece) -> (arena off, part stride)
+    for (const Group &gr : groups) {
+        const GenRef &ref = gens[(size_t)gr.g];
+        const size_t sbytes = (size_t)ref.n * (size_t)(ref.bits / 8);
+        const size_t nb = gr.pieces.size();
+        const int P = std::min(parts, gr.nchunks);
+        const size_t base = arena_bytes;
+        for (size_t b = 0; b < nb; b++)
+            slot_of[{gr.g, gr.pieces[b]}] = {base + b * sbytes, P};
+        for (int q = 0; q < P; q++) {
+            JumpItem ji{};
+            ji.state_in = (uint64_t)(uintptr_t)ref.state_in;
+            ji.masks = gr.mask_off;                         // fixed up below
+            ji.states_out = base + (size_t)q * nb * sbytes;  // fixed up below
+            ji.nb = (int32_t)nb;
+            ji.c0 = (int32_t)((long)gr.nchunks * q / P);
+            ji.c1 = (int32_t)((long)gr.nchunks * (q + 1) / P);
+            ji.param = ref.slot;
+            ji.state_bytes = (int32_t)sbytes;
+            jitems.push_back(ji);
+            jgroup.push_back((ref.bits == 64 ? 3 : 0) + kernel_class(ref.bits, ref.n));
+        }
+        arena_bytes += (((size_t)P * nb * sbytes + 255) & ~(size_t)255);
+    }
+    for (int g = 0; g < G; g++) {
+        const GenRef &gr = gens[(size_t)g];
+        const size_t sbytes = (size_t)gr.n * (size_t)(gr.bits / 8);
         for (size_t k = 0; k < pieces[(size_t)g].size(); k++) {
             const Piece &p = pieces[(size_t)g][k];
             Item it{};
@@ -504,10 +572,18 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             it.state_out = (p.off + p.len == total[(size_t)g]) ? (uint64_t)(uintptr_t)gr.state_out : 0;
             it.param = gr.slot;
             it.poly = UINT64_MAX;
+            uint64_t slot = 0;
             if (p.off > 0) {
                 if (jump_mode == 1) {
-                    it.state_in = UINT64_MAX - 1;  // arena slot, fixed up below
-                    it.pad2 = slot_of_piece[k];
+                    const auto &sp = slot_of.at({g, k});
+                    slot = sp.first + 1;
+                    it.parts = sp.second;
+                    // partial windows of part q sit q * nb * sbytes apart
+                    size_t nb = 0;
+                    for (const Group &grp : groups)
+                        if (grp.g == g && std::find(grp.pieces.begin(), grp.pieces.end(), k) != grp.pieces.end())
+                            nb = grp.pieces.size();
+                    it.part_bytes = nb * sbytes;
                 } else {
                     const auto &pw = polys[(size_t)g][k];
                     it.poly = poly_host.size();  // word offset, fixed up below
@@ -516,6 +592,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
                 }
             }
             list.push_back(it);
+            arena_slot.push_back(slot);
             group.push_back((gr.bits == 64 ? 3 : 0) + kernel_class(gr.bits, gr.n));
             if (it.state_out)
                 commits.push_back(g);
@@ -551,10 +628,8 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     for (int i : order) {
         Item it = list[(size_t)i];
         it.poly = it.poly == UINT64_MAX ? 0 : (uint64_t)(uintptr_t)(d_polys + it.poly);
-        if (it.state_in == UINT64_MAX - 1) {
-            it.state_in = (uint64_t)(uintptr_t)d_jstates + it.pad2;
-            it.pad2 = 0;
-        }
+        if (arena_slot[(size_t)i])
+            it.state_in = (uint64_t)(uintptr_t)d_jstates + arena_slot[(size_t)i] - 1;
         items.push_back(it);
     }
     // Jump launches, one per (word size, class), before the generation.
@@ -580,6 +655,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             L.first = (int)i;
             L.count = (int)(j - i);
             L.grid = std::min(L.count, c.cus * c.jump_blocks_per_cu[L.bits == 64][L.cls]);
+            L.dyn_lds = 0;
             jlaunches.push_back(L);
             i = j;
         }
@@ -601,8 +677,9 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         L.layout = c.layout[L.bits == 64][L.cls];
         L.first = (int)i;
         L.count = (int)(j - i);
-        const int cap = c.cus * std::min(c.default_bpc, c.blocks_per_cu[L.bits == 64][L.cls][L.layout]);
-        L.grid = std::min(L.count, cap);
+        const int bpc = std::min(c.default_bpc, c.blocks_per_cu[L.bits == 64][L.cls][L.layout]);
+        L.grid = std::min(L.count, c.cus * bpc);
+        L.dyn_lds = c.lds_reserve(kernel_fn(L.bits, L.cls, L.layout), bpc);
         launches.push_back(L);
         i = j;
     }
@@ -627,13 +704,14 @@ void Plan::launch_part(int part, hipStream_t s) {
     const size_t jq = 2 * launches.size();
     for (size_t k = 0; part == 0 && k < jlaunches.size(); k++) {
         const Launch &L = jlaunches[k];
-        hip_check(launch_jumps(L.bits, L.cls, d_jitems + L.first, L.count, ctx->d_params, d_queue + jq + 2 * k, L.grid, s),
+        hip_check(launch_jumps(L.bits, L.cls, d_jitems + L.first, L.count, ctx->d_params, d_queue + jq + 2 * k, L.grid,
+                               L.dyn_lds, s),
                   "jump kernel launch");
     }
     for (size_t k = 0; part == 1 && k < launches.size(); k++) {
         const Launch &L = launches[k];
         hip_check(launch_items(L.bits, L.cls, L.layout, d_items + L.first, L.count, ctx->d_params, d_queue + 2 * k,
-                               L.grid, s),
+                               L.grid, L.dyn_lds, s),
                   "kernel launch");
     }
 }
@@ -646,9 +724,9 @@ void Plan::configure(int layout, int bpc) {
             L.layout = layout;
         }
         int cap = ctx->blocks_per_cu[L.bits == 64][L.cls][L.layout];
-        if (bpc > 0)
-            cap = std::min(cap, bpc);
+        cap = std::min(cap, bpc > 0 ? bpc : ctx->default_bpc);
         L.grid = std::min(L.count, ctx->cus * cap);
+        L.dyn_lds = ctx->lds_reserve(kernel_fn(L.bits, L.cls, L.layout), cap);
     }
 }
 
@@ -731,7 +809,8 @@ static void device_jump(Context &c, int slot, int bits, int mexp, const void *in
     const std::vector<uint32_t> words = gf2::to_words32(P, std::max<int>(1, (int)((strlen(hex) * 4 + 31) / 32)));
     std::vector<uint64_t> masks;
     JumpItem ji{};
-    ji.nchunks = append_jump_masks({&words}, masks);
+    ji.c0 = 0;
+    ji.c1 = append_jump_masks({&words}, masks);
     ji.state_in = (uint64_t)(uintptr_t)in;
     ji.states_out = (uint64_t)(uintptr_t)out;
     ji.nb = 1;
@@ -753,7 +832,7 @@ static void device_jump(Context &c, int slot, int bits, int mexp, const void *in
         hip_check(hipMemcpy(d_item, &ji, sizeof(JumpItem), hipMemcpyHostToDevice), "hipMemcpy");
         hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
         hip_check(hipMemset(d_q, 0, 8), "hipMemset");
-        hip_check(launch_jumps(bits, kernel_class(bits, state_words(bits, mexp)), d_item, 1, c.d_params, d_q, 1, 0),
+        hip_check(launch_jumps(bits, kernel_class(bits, state_words(bits, mexp)), d_item, 1, c.d_params, d_q, 1, 0, 0),
                   "jump kernel launch");
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (...) {
@@ -1263,7 +1342,8 @@ int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) 
             JumpItem ji{};
             ji.state_in = (uint64_t)(uintptr_t)d_base;
             ji.masks = masks.size() * 8;
-            ji.nchunks = append_jump_masks(ps, masks);
+            ji.c0 = 0;
+            ji.c1 = append_jump_masks(ps, masks);
             ji.states_out = (uint64_t)(uintptr_t)s->state(g0, 0);
             ji.nb = nb;
             ji.param = s->slot[0];
@@ -1281,7 +1361,7 @@ int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) 
         const int bits = s->bits[0], cls = kernel_class(bits, s->n[0]);
         const int n_items = (int)items.size();
         const int grid = std::min(n_items, c.cus * c.jump_blocks_per_cu[bits == 64][cls]);
-        hip_check(launch_jumps(bits, cls, d_items, n_items, c.d_params, d_q, grid, 0), "jump kernel launch");
+        hip_check(launch_jumps(bits, cls, d_items, n_items, c.d_params, d_q, grid, 0, 0), "jump kernel launch");
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (...) {
         cleanup();

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -45,6 +46,10 @@ struct Context {
     std::vector<Custom> customs;
     std::mutex mu;
 
+    std::map<std::pair<const void *, int>, size_t> reserve_cache;
+    // Dynamic LDS that limits `fn` to `per_cu` resident workgroups per CU
+    // (0 when the kernel's own limit is already at most per_cu).
+    size_t lds_reserve(const void *fn, int per_cu);
     int slot_of(const DevParam &rec, const unsigned char *sha1);
     const DevParam &param(int slot) const;
     void sha1_of(int slot, unsigned char out[20]) const;
@@ -78,6 +83,7 @@ uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, 
 struct Plan {
     struct Launch {
         int bits, cls, layout, first, count, grid;
+        size_t dyn_lds;
     };
     Context *ctx = nullptr;
     // Start states of segments: 0 = jump inside the generation kernel

@@ -11,7 +11,7 @@ extern "C" int emu_run_items(int bits, int cls, int layout, const mtgp::Item *it
         for (int k = 0; k < mtgp::kNumSets; k++)
             params.push_back(mtgp::host_param(k));
     uint32_t queue[2] = {0, 0};
-    if (mtgp::launch_items(bits, cls, layout, items, n_items, params.data(), queue, grid, nullptr) != hipSuccess)
+    if (mtgp::launch_items(bits, cls, layout, items, n_items, params.data(), queue, grid, 0, nullptr) != hipSuccess)
         return -1;
     return (queue[0] == 0 && queue[1] == 0) ? 0 : -2;  // the counters must reset themselves
 }
@@ -22,7 +22,7 @@ extern "C" int emu_run_jumps(int bits, int cls, const mtgp::JumpItem *items, int
         for (int k = 0; k < mtgp::kNumSets; k++)
             params.push_back(mtgp::host_param(k));
     uint32_t queue[2] = {0, 0};
-    if (mtgp::launch_jumps(bits, cls, items, n_items, params.data(), queue, grid, nullptr) != hipSuccess)
+    if (mtgp::launch_jumps(bits, cls, items, n_items, params.data(), queue, grid, 0, nullptr) != hipSuccess)
         return -1;
     return (queue[0] == 0 && queue[1] == 0) ? 0 : -2;
 }

@@ -46,6 +46,7 @@ inline T emu_lane_value(T v, uint32_t src) {
     return r;
 }
 #define __builtin_amdgcn_readfirstlane(v) emu_lane_value<uint32_t>((uint32_t)(v), 0u)
+#define __builtin_amdgcn_readlane(v, src) emu_lane_value<uint32_t>((uint32_t)(v), (uint32_t)(src))
 template <class T>
 inline T __shfl(T v, int src) { return emu_lane_value<T>(v, (uint32_t)src); }
 

@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ITEM = np.dtype([("out", "<u8"), ("count", "<u8"), ("state_in", "<u8"), ("state_out", "<u8"),
                  ("poly", "<u8"), ("poly_words", "<i4"), ("param", "<i4"), ("kind", "<i4"),
-                 ("pad", "<i4"), ("pad2", "<u8")])
+                 ("parts", "<i4"), ("part_bytes", "<u8")])
 assert ITEM.itemsize == 64
 
 
@@ -24,7 +24,8 @@ def load():
 
 
 JUMP_ITEM = np.dtype([("state_in", "<u8"), ("masks", "<u8"), ("states_out", "<u8"), ("nb", "<i4"),
-                      ("nchunks", "<i4"), ("param", "<i4"), ("state_bytes", "<i4"), ("pad", "<u8", (3,))])
+                      ("c0", "<i4"), ("c1", "<i4"), ("param", "<i4"), ("state_bytes", "<i4"), ("pad0", "<i4"),
+                      ("pad", "<u8", (2,))])
 assert JUMP_ITEM.itemsize == 64
 BATCH, CHUNK = 4, 1024
 

@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ from the reference itself.
+
+Everything here comes from the reference library compiled from its own
+sources (oracle/_ref, built by `make -C oracle ref`) or from data files the
+reference ships (openCL-sample/mtgp32-jump-string.h, mtgp32-jump-table.cpp,
+cuda-sample/mtgp32-char11213.h).  No output of this repository's code is
+used.  Run in the container that has the reference tree:
+
+    make -C oracle ref && python3 tests/golden/make_golden.py
+
+Files written (all small, loaded with numpy allow_pickle=False or json):
+  params_digests.json   SHA-256 of every full parameter record (768 sets)
+                        and two full records per table
+  main_outputs.npz      the reference's own test-program sequences
+                        (mtgp*-fast.c MAIN: init(1), init_by_array(1,2,3,4),
+                        init_by_str("\\1\\2\\3\\4") + the float family)
+  jump_steps.npz        test-jump32.cpp pattern: outputs after stepping s
+  reference_jump_data.npz  CHAR32_11213, JUMP3_162, mtgp32_jump_table[6][351]
+  streams.json          SHA-256 digests of the benchmark-shaped streams
+  substreams.npz        substreams i*3^162 of set 0, seed 777
+"""
+import hashlib
+import json
+import os
+import re
+import sys
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, ROOT)
+REF = "/root/reference"
+
+from pyoracle import RefLib  # noqa: E402
+
+MEXPS = {32: (11213, 23209, 44497), 64: (23209, 44497, 110503)}
+MASK64 = (1 << 64) - 1
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def splitmix64(state, count):
+    out = []
+    for _ in range(count):
+        state = (state + 0x9E3779B97F4A7C15) & MASK64
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def params_digests():
+    out = {"digests": {}, "records": {}}
+    for bits, mexps in MEXPS.items():
+        ref = RefLib("fast", bits)
+        for mexp in mexps:
+            ds = []
+            for i in range(128):
+                rec = ref.params(mexp, i)
+                ds.append(sha(rec))
+                if i in (0, 127):
+                    out["records"]["%d/%d/%d" % (bits, mexp, i)] = [int(v) for v in rec]
+            out["digests"]["%d/%d" % (bits, mexp)] = ds
+    with open(os.path.join(HERE, "params_digests.json"), "w") as f:
+        json.dump(out, f)
+
+
+def main_outputs():
+    arrays = {}
+    for bits, mexps in MEXPS.items():
+        ref = RefLib("fast", bits)
+        for mexp in mexps:
+            for idx in (0, 127):
+                key = "%d_%d_%d" % (bits, mexp, idx)
+                g = ref.new_seed(mexp, idx, 1)
+                arrays[key + "_init"] = g.fill(0, 1000)
+                g = ref.new_array(mexp, idx, [1, 2, 3, 4])
+                arrays[key + "_array"] = g.fill(0, 1000)
+                g = ref.new_str(mexp, idx, b"\x01\x02\x03\x04")
+                arrays[key + "_str"] = g.fill(0, 1000)
+                for kind in (1, 2, 3, 4):
+                    arrays[key + "_str_k%d" % kind] = g.fill(kind, 1000)
+                # a string with bytes >= 0x80 (init_by_str adds them as signed char)
+                g = ref.new_str(mexp, idx, b"MTGP\xff\x80\x7f!")
+                arrays[key + "_str_signed"] = g.fill(0, 256)
+    np.savez_compressed(os.path.join(HERE, "main_outputs.npz"), **arrays)
+
+
+def jump_steps():
+    """After mtgp_init(seed 0) and three draws, step s numbers and record the
+    next 100 (the comparison test-jump32.cpp:218-264 makes after jumping)."""
+    arrays = {}
+    for bits, mexps in MEXPS.items():
+        ref = RefLib("fast", bits)
+        for mexp in mexps:
+            if mexp == 110503:
+                continue
+            n = mexp // bits + 1
+            for s in (1, 2, n + 1, 32 * n - 1, 32 * n + 1, 3003, 200004, 10000005):
+                g = ref.new_seed(mexp, 0, 0)
+                g.fill(0, 3)
+                g.fill(0, s)
+                arrays["%d_%d_%d" % (bits, mexp, s)] = g.fill(0, 100)
+    np.savez_compressed(os.path.join(HERE, "jump_steps.npz"), **arrays)
+
+
+def reference_jump_data():
+    char = re.search(r'"([0-9a-f]+)"', open(REF + "/cuda-sample/mtgp32-char11213.h").read()).group(1)
+    js = re.search(r'"([0-9a-f]+)"', open(REF + "/openCL-sample/mtgp32-jump-string.h").read()).group(1)
+    body = open(REF + "/openCL-sample/mtgp32-jump-table.cpp").read()
+    body = body[body.index("mtgp32_jump_table[] = {"):]
+    vals = [int(v, 16) for v in re.findall(r"0x([0-9a-fA-F]{8})", body)]
+    assert len(vals) == 6 * 351, len(vals)
+    np.savez_compressed(os.path.join(HERE, "reference_jump_data.npz"),
+                        char32_11213=np.array(char), jump3_162=np.array(js),
+                        jump_table=np.array(vals, np.uint32).reshape(6, 351))
+
+
+def _batched32(g, seed, L):
+    ref = RefLib("fast", 32)
+    gen = ref.new_seed(11213, g, seed)
+    a = gen.fill(0, L)
+    b = gen.fill(2, L)
+    return g, sha(a), sha(b), sha(gen.window())
+
+
+def _batched64(mexp, i, key):
+    ref = RefLib("fast", 64)
+    gen = ref.new_array(mexp, i, key)
+    a = gen.fill(0, 1 << 24)
+    b = gen.fill(2, 1 << 24)
+    return mexp, i, sha(a), sha(b), sha(gen.window())
+
+
+def streams():
+    import bench
+    out = {}
+    # plain MTGP32-11213 set 0, seed 1234: 2^24 uint32 then 2^24 close1_open2
+    ref = RefLib("fast", 32)
+    g = ref.new_seed(11213, 0, 1234)
+    a = g.fill(0, 1 << 24)
+    b = g.fill(1, 1 << 24)
+    out["plain"] = {"uint32": sha(a), "close1_open2": sha(b), "window": g.window().tolist(),
+                    "uint32_head": a[:16].tolist(), "close1_open2_head": b[:16].tolist()}
+    seeds, lengths = bench.batched_workload()
+    with ProcessPoolExecutor(8) as ex:
+        res = list(ex.map(_batched32, range(128), seeds, lengths))
+    out["batched32"] = {"seeds": seeds, "lengths": lengths,
+                        "uint32": [r[1] for r in res], "close_open": [r[2] for r in res],
+                        "window": [r[3] for r in res]}
+    jobs = []
+    for mexp in (23209, 44497):
+        for i in range(64):
+            key = splitmix64((mexp << 8) | i, 4)
+            jobs.append((mexp, i, key))
+    with ProcessPoolExecutor(8) as ex:
+        res = list(ex.map(_batched64, *zip(*jobs)))
+    out["batched64"] = {"keys": {"%d/%d" % (m, i): [str(k) for k in key] for m, i, key in jobs},
+                        "uint64": {"%d/%d" % (r[0], r[1]): r[2] for r in res},
+                        "close_open": {"%d/%d" % (r[0], r[1]): r[3] for r in res},
+                        "window": {"%d/%d" % (r[0], r[1]): r[4] for r in res}}
+    with open(os.path.join(HERE, "streams.json"), "w") as f:
+        json.dump(out, f)
+
+
+def substreams():
+    """Substream i of set 0, seed 777 = mtgp32_fast_jump(JUMP3_162) applied
+    i times (as the reference's jump samples check it)."""
+    js = re.search(r'"([0-9a-f]+)"', open(REF + "/openCL-sample/mtgp32-jump-string.h").read()).group(1)
+    ref = RefLib("fast", 32)
+    sample = [0, 1, 2, 3, 7, 64, 1000, 4095]
+    arrays = {"indices": np.array(sample)}
+    g = ref.new_seed(11213, 0, 777)
+    base = g.window()
+    cur = 0
+    for i in sample:
+        while cur < i:
+            g.jump(js)
+            cur += 1
+        h = ref.new_seed(11213, 0, 777)
+        h.set_window(g.window())
+        arrays["window_%d" % i] = h.window()
+        out = h.fill(0, 1 << 20)
+        arrays["head_%d" % i] = out[:1024]
+        arrays["sha_%d" % i] = np.array(sha(out))
+    arrays["base"] = base
+    np.savez_compressed(os.path.join(HERE, "substreams.npz"), **arrays)
+
+
+if __name__ == "__main__":
+    for step in (params_digests, main_outputs, jump_steps, reference_jump_data, substreams, streams):
+        print("writing", step.__name__, flush=True)
+        step()

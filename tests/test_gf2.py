@@ -1,0 +1,89 @@
+"""The library's NTL-free GF(2)[x] service (charpoly by Berlekamp-Massey,
+jump polynomials x^s mod P) against the reference's shipped jump data, the
+table's poly_sha1 and the oracle.  Host arithmetic only: no GPU."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _coeffs(hexs):
+    bits = []
+    for ch in hexs:
+        v = int(ch, 16)
+        bits.extend((v >> b) & 1 for b in range(4))
+    while bits and not bits[-1]:
+        bits.pop()
+    return bits
+
+
+def test_charpoly_is_shipped_characteristic(mtgp):
+    jd = np.load(os.path.join(GOLD, "reference_jump_data.npz"))
+    assert mtgp.charpoly_hex(32, 11213, 0) == str(jd["char32_11213"])
+
+
+@pytest.mark.parametrize("bits,mexp,sets", [(32, 11213, range(128)), (32, 23209, (0, 64, 127)),
+                                             (32, 44497, (5,)), (64, 23209, (0, 31, 127)), (64, 44497, (9,))])
+def test_charpoly_sha1_matches_table(mtgp, bits, mexp, sets):
+    """The library checks this itself (it refuses a mismatch); verify the
+    digest independently with hashlib."""
+    for i in sets:
+        c = _coeffs(mtgp.charpoly_hex(bits, mexp, i))
+        assert len(c) == mexp + 1
+        s = "".join(map(str, c))
+        assert hashlib.sha1(s.encode()).hexdigest() == bytes(mtgp.params(bits, mexp, i).poly_sha1)[:20].hex()
+
+
+def test_jump_string_and_table(mtgp, oracle):
+    jd = np.load(os.path.join(GOLD, "reference_jump_data.npz"))
+    assert mtgp.jump_hex(32, 11213, 0, 3**162) == str(jd["jump3_162"])
+    for k in range(6):
+        row = jd["jump_table"][k]
+        h = mtgp.jump_hex(32, 11213, 0, 4**k * 3**162)
+        words = 11213 // 64 + 2
+        got = oracle.hex_to_poly(h, words)
+        want = np.frombuffer(row.tobytes() + b"\0" * (8 * words - 4 * 351), np.uint64)
+        assert (got == want).all(), k
+
+
+def test_jump_matches_oracle_arithmetic(mtgp, oracle):
+    """x^s mod P from the library equals the oracle's schoolbook powering,
+    for s across sizes including 2^64 and 2^64 + 12345."""
+    P = oracle.hex_to_poly(mtgp.charpoly_hex(32, 23209, 7), 23209 // 64 + 2)
+    for s in (0, 1, 23208, 23209, 23210, 10**9 + 7, 2**64, 2**64 + 12345, 3**100):
+        got = mtgp.jump_hex(32, 23209, 7, s)
+        assert got == oracle.poly_to_hex(oracle.xpow_mod(s, P)), s
+
+
+def test_jump_poly_then_oracle_jump_equals_stepping(mtgp, oracle):
+    """test-jump32.cpp pattern with the library's jump polynomials and the
+    oracle's literal jump, against golden outputs of the stepped reference."""
+    gold = np.load(os.path.join(GOLD, "jump_steps.npz"))
+    for bits, mexp in [(32, 23209), (32, 44497), (64, 44497)]:
+        p = oracle.params(bits, mexp, 0)
+        _, w = oracle.generate(p, oracle.init_state(p, 0), 0, 3)
+        n = p.n
+        for s in (n + 1, 3003, 10000005):
+            wj = oracle.jump(p, w, mtgp.jump_hex(bits, mexp, 0, s))
+            got, _ = oracle.generate(p, wj, 0, 100)
+            assert (got == gold["%d_%d_%d" % (bits, mexp, s)]).all(), (bits, mexp, s)
+
+
+def test_inconsistent_parameter_set_is_refused(mtgp):
+    """A record whose poly_sha1 does not match its recursion is refused
+    (the characteristic polynomial is checked before it is used)."""
+    import ctypes as C
+    p = mtgp.Params32.from_buffer_copy(mtgp.params(32, 11213, 3))
+    p.poly_sha1[0] ^= 1
+    buf = C.create_string_buffer(4096)
+    assert mtgp.lib.mtgp32_charpoly_hex(C.byref(p), buf, len(buf)) == -1
+    assert "poly_sha1" in mtgp.last_error()
+    e = (C.c_uint64 * 1)(12345)
+    assert mtgp.lib.mtgp32_jump_hex(C.byref(p), e, 1, buf, len(buf)) == -1
+    # a buffer too small is an error, not an overflow
+    good = mtgp.params(32, 11213, 3)
+    small = C.create_string_buffer(10)
+    assert mtgp.lib.mtgp32_jump_hex(C.byref(good), e, 1, small, len(small)) == -1

@@ -116,22 +116,46 @@ def test_batched_jumps(emu, oracle, bits, mexp):
             top = mexp - 1 if b != 2 else mexp + 700
             cl.append(sorted(set(rng.integers(0, top + 1, size=top // 2).tolist()) | {top}))
         jobs.append(cl)
-    arr = np.zeros(len(jobs), pyemu.JUMP_ITEM)
-    keep = []
+    # each job runs as `parts` items over disjoint chunk ranges; their
+    # partial windows XOR to the jump (the generation kernel's Item.parts)
+    items, keep = [], []
     for k, cl in enumerate(jobs):
         masks, nchunks = pyemu.jump_masks(cl)
-        outs = np.zeros((len(cl), p.n), wd)
+        parts = min(k + 1, nchunks)
+        outs = np.zeros((parts, len(cl), p.n), wd)
         keep.append((masks, outs))
-        arr[k]["state_in"] = base.ctypes.data
-        arr[k]["masks"] = masks.ctypes.data
-        arr[k]["states_out"] = outs.ctypes.data
-        arr[k]["nb"] = len(cl)
-        arr[k]["nchunks"] = nchunks
-        arr[k]["param"] = pyemu.set_id(bits, mexp, 11)
-        arr[k]["state_bytes"] = p.n * (bits // 8)
-    rc = emu.emu_run_jumps(bits, pyemu.kernel_class(bits, p.n), arr.ctypes.data, len(jobs), 2)
+        for q in range(parts):
+            items.append(dict(state_in=base.ctypes.data, masks=masks.ctypes.data,
+                              states_out=outs[q].ctypes.data, nb=len(cl), c0=nchunks * q // parts,
+                              c1=nchunks * (q + 1) // parts, param=pyemu.set_id(bits, mexp, 11),
+                              state_bytes=p.n * (bits // 8)))
+    arr = np.zeros(len(items), pyemu.JUMP_ITEM)
+    for k, it in enumerate(items):
+        for f, v in it.items():
+            arr[k][f] = v
+    rc = emu.emu_run_jumps(bits, pyemu.kernel_class(bits, p.n), arr.ctypes.data, len(items), 2)
     assert rc == 0
     for (masks, outs), cl in zip(keep, jobs):
+        got = np.bitwise_xor.reduce(outs, axis=0)
         for b, cs in enumerate(cl):
             want = oracle.jump(p, base, _hex_of(cs, cs[-1]))
-            assert (outs[b] == want).all(), "jump %d of %d" % (b, len(cl))
+            assert (got[b] == want).all(), "jump %d of %d" % (b, len(cl))
+
+
+@pytest.mark.parametrize("bits,mexp", [(32, 11213), (64, 23209)])
+def test_item_start_window_from_parts(emu, oracle, bits, mexp):
+    """Item.parts: the start window is the XOR of partial windows."""
+    p = oracle.params(bits, mexp, 4)
+    w = oracle.init_state(p, 8)
+    rng = np.random.default_rng(3)
+    wd = np.uint32 if bits == 32 else np.uint64
+    parts = rng.integers(0, np.iinfo(wd).max, (3, p.n), dtype=wd)
+    parts[2] = w ^ parts[0] ^ parts[1]
+    n = 777
+    out = np.zeros(n, wd)
+    sout = np.zeros_like(w)
+    _run(emu, bits, [dict(out=out.ctypes.data, count=n, state_in=parts.ctypes.data, state_out=sout.ctypes.data,
+                          param=pyemu.set_id(bits, mexp, 4), kind=0, parts=3, part_bytes=p.n * bits // 8,
+                          n=p.n)], 1)
+    e, we = oracle.generate(p, w, 0, n)
+    assert (out == e).all() and (sout == we).all()
