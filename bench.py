@@ -8,11 +8,12 @@ Workload (the batched configuration the metric is quoted on):
   * generator g emits L_g uint32 (genrand_uint32) followed by L_g floats in
     [0,1) (genrand_close_open), L_g = floor(2^U_g), U_g = 20 + 6 u_g with
     u_g the (g+1)-th output of splitmix64 started at 7, scaled to [0,1);
-  * all of it in ONE persistent kernel launch per step (the plan splits each
-    stream into segments whose start states are reached by device jump).
+  * each stream is split into segments; one batched jump-ahead launch
+    computes the segment start states, then ONE persistent generation launch
+    writes every output.
 
-A step is one launch of that plan (inputs -- states, parameter tables, work
-list -- resident in HBM).  value = output bytes of all steps / time of the
+A step is one launch of that plan: the jump launch plus the generation
+launch (inputs -- states, parameter tables, work list -- resident in HBM).  value = output bytes of all steps / time of the
 timed steps.  Bit-exactness is checked on samples of every step's output
 against the CPU oracle, and the CPU baseline runs the reference library
 (compiled from its own sources) on the host cores.
@@ -77,19 +78,27 @@ def run_gpu(args):
         plan.launch(stream)
     stream.synchronize()
     M.synchronize()
-    ev = [M.Event() for _ in range(args.steps + 1)]
+    # One step = the jump launch (segment start states) + the generation
+    # launch, on one stream.  Events between them time each kernel on the
+    # stream it runs on.
+    ev = [M.Event() for _ in range(2 * args.steps + 1)]
     ev[0].record(stream)
     for k in range(args.steps):
-        plan.launch(stream)
-        ev[k + 1].record(stream)
+        plan.launch_part(0, stream)
+        ev[2 * k + 1].record(stream)
+        plan.launch_part(1, stream)
+        ev[2 * k + 2].record(stream)
     stream.synchronize()
     M.synchronize()
-    per = [ev[k].elapsed_ms(ev[k + 1]) for k in range(args.steps)]
+    per = [ev[2 * k].elapsed_ms(ev[2 * k + 2]) for k in range(args.steps)]
+    jump_ms = [ev[2 * k].elapsed_ms(ev[2 * k + 1]) for k in range(args.steps)]
+    gen_ms = [ev[2 * k + 1].elapsed_ms(ev[2 * k + 2]) for k in range(args.steps)]
     total_ms = ev[0].elapsed_ms(ev[-1])
     # Bit-exact spot check of the last step's output against the oracle.
     exact = spot_check(M, buf, seeds, lengths) if not args.no_check else None
     return dict(G=G, cus=cus, hbm=hbm, out_bytes=out_bytes, numbers=total_numbers, per_ms=per,
-                total_ms=total_ms, stats=stats, plan_s=time.time() - t0, exact=exact)
+                jump_ms=jump_ms, gen_ms=gen_ms, total_ms=total_ms, stats=stats, plan_s=time.time() - t0,
+                exact=exact)
 
 
 def spot_check(M, buf, seeds, lengths, n=1 << 16):
@@ -116,42 +125,82 @@ def spot_check(M, buf, seeds, lengths, n=1 << 16):
     return ok
 
 
-def cpu_baseline(seconds_target=15.0):
-    """The reference library (oracle/_ref, built from the reference sources)
-    on the host cores: one thread per generator of the same workload, each
-    producing 2^24 uint32 then 2^24 close_open floats."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_run(threads, seconds, chunk):
+    """Numbers generated by `threads` host threads in `seconds`: thread t
+    runs generator t of the workload (MTGP32-11213 set t, its seed) through
+    the reference library (oracle/_ref, compiled from the reference's own
+    sources; the C restatement in oracle/ when that build is absent),
+    alternating chunks of uint32 and close_open floats into fresh arrays."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Oracle, RefLib
     seeds, _ = batched_workload()
-    threads = max(1, min(16, os.cpu_count() or 1))
-    per = 1 << 24
     kind = "reference" if RefLib.available("fast", 32) else "port"
+    done = [0] * threads
     if kind == "reference":
         ref = RefLib("fast", 32)
         gens = [ref.new_seed(11213, g, seeds[g]) for g in range(threads)]
 
-        def work(g):
-            gens[g].fill(0, per)
-            gens[g].fill(2, per)
+        def step(g, k):
+            gens[g].fill(0 if k % 2 == 0 else 2, chunk)
     else:
         o = Oracle()
         ps = [o.params(32, 11213, g) for g in range(threads)]
         ws = [o.init_state(ps[g], seeds[g]) for g in range(threads)]
 
-        def work(g):
-            _, w = o.generate(ps[g], ws[g], 0, per)
-            o.generate(ps[g], w, 2, per)
+        def step(g, k):
+            _, ws[g] = o.generate(ps[g], ws[g], 0 if k % 2 == 0 else 2, chunk)
+    deadline = time.time() + seconds
+
+    def work(g):
+        k = 0
+        while time.time() < deadline:
+            step(g, k)
+            done[g] += chunk
+            k += 1
     th = [threading.Thread(target=work, args=(g,)) for g in range(threads)]
     t = time.time()
     for x in th:
         x.start()
     for x in th:
         x.join()
-    dt = time.time() - t
-    nbytes = threads * 2 * per * 4
-    return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": kind,
-            "sample": "%d generators (MTGP32-11213 sets 0..%d, same seeds) x (2^24 uint32 + 2^24 close_open), "
-                      "one host thread each, %.1f s wall" % (threads, threads - 1, dt)}
+    return kind, sum(done), time.time() - t
+
+
+def cpu_baseline(seconds=8.0, chunk=1 << 22):
+    """The reference CPU path on the host cores, on a time-bounded sample:
+    one core for `seconds`, then one thread per core (at most 16) for
+    `seconds`.  `value` is the all-core rate."""
+    cores = max(1, min(16, os.cpu_count() or 1))
+    _, n1, t1 = _cpu_run(1, seconds, chunk)
+    kind, n, dt = _cpu_run(cores, seconds, chunk)
+    return {"value": round(4 * n / dt / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": kind,
+            "single_core_gbs": round(4 * n1 / t1 / 1e9, 3), "nproc": os.cpu_count(), "cpu": _cpu_model(),
+            "sample": "MTGP32-11213 sets 0..%d of the workload with their seeds, one host thread each, "
+                      "alternating 2^22 uint32 / 2^22 close_open chunks: %.3g numbers in %.1f s wall on %d "
+                      "threads (single core: %.3g numbers in %.1f s)" % (cores - 1, n, dt, cores, n1, t1)}
+
+
+def measured_traffic(path=os.path.join(ROOT, "profiles", "mtgp32_items_batched11213_pmc.txt")):
+    """HBM bytes per generation launch from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each scaled
+    by its copy-kernel calibration factor); None when the summary is absent."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -168,9 +217,13 @@ def main():
               "path has no cross-GPU step; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     r = run_gpu(args)
-    avg_ms = sum(r["per_ms"]) / len(r["per_ms"])
     gbs = r["out_bytes"] * args.steps / (r["total_ms"] * 1e-3) / 1e9
-    kern_gbs = r["out_bytes"] / (avg_ms * 1e-3) / 1e9
+    # Dominant kernel: the generation launch.  Algorithmic bytes are the
+    # output bytes (4 per uint32 / float); start windows and tables, under
+    # 0.01%, are left out.
+    gen_avg = sum(r["gen_ms"]) / len(r["gen_ms"])
+    alg_bytes = r["out_bytes"]
+    kern_gbs = alg_bytes / (gen_avg * 1e-3) / 1e9
     line = {
         "metric": METRIC,
         "value": round(gbs, 2),
@@ -188,12 +241,15 @@ def main():
         "numbers_per_s": round(r["numbers"] * args.steps / (r["total_ms"] * 1e-3), 1),
         "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
         "config": {"workload": "batched MTGP32-11213: 128 parameter sets, uneven lengths 2^20..2^26 "
-                               "uint32 + close_open float32 each, one persistent launch",
+                               "uint32 + close_open float32 each; per step one jump launch + one persistent generation launch",
                    "generators": r["G"], "output_bytes": r["out_bytes"], "numbers": r["numbers"],
                    "work_items": r["stats"]["items"], "launches_per_step": r["stats"]["launches"],
-                   "plan_ms": round(r["stats"]["plan_ms"], 1), "compute_units": r["cus"]},
-        "roofline": {"bound": "hbm", "achieved": round(kern_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(kern_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                   "plan_ms": round(r["stats"]["plan_ms"], 1), "compute_units": r["cus"],
+                   "jump_ms": round(sum(r["jump_ms"]) / len(r["jump_ms"]), 4),
+                   "generate_ms": round(gen_avg, 4)},
+        "roofline": {"bound": "hbm", "kernel": "mtgp32_items_kernel<351,3,1>", "achieved": round(kern_gbs, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(kern_gbs / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes": alg_bytes, "traffic": measured_traffic()},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

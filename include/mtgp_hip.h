@@ -52,6 +52,9 @@ void *mtgp_dev_malloc(size_t bytes);
 int mtgp_dev_free(void *ptr);
 int mtgp_memcpy_htod(void *dst, const void *src, size_t bytes);
 int mtgp_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (enqueued on `stream`,
+ * NULL = default stream).  Reference point for hardware byte counters. */
+int mtgp_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 int mtgp_synchronize(void);
 /* Streams and events for callers without their own HIP code (timing). */
 void *mtgp_stream_create(void);

@@ -100,6 +100,7 @@ _sig("mtgp_dev_malloc", vp, sz)
 _sig("mtgp_dev_free", i32, vp)
 _sig("mtgp_memcpy_htod", i32, vp, vp, sz)
 _sig("mtgp_memcpy_dtoh", i32, vp, vp, sz)
+_sig("mtgp_device_copy", i32, vp, vp, sz, vp)
 _sig("mtgp_synchronize", i32)
 _sig("mtgp_stream_create", vp)
 _sig("mtgp_stream_destroy", i32, vp)
@@ -206,6 +207,13 @@ class DeviceBuffer:
 
 def synchronize():
     _check(_lib.mtgp_synchronize(), "mtgp_synchronize")
+
+
+def device_copy(dst, src, nbytes, stream=None):
+    """dst <- src (DeviceBuffers or device addresses) by a plain copy kernel."""
+    d = dst.ptr if isinstance(dst, DeviceBuffer) else dst
+    s = src.ptr if isinstance(src, DeviceBuffer) else src
+    _check(_lib.mtgp_device_copy(d, s, int(nbytes), stream.handle if stream else None), "device_copy")
 
 
 class Stream:

@@ -1048,4 +1048,22 @@ hipError_t launch_jumps(int bits, int cls, const JumpItem *items, int n_items, c
     });
 }
 
+namespace {
+// Plain streaming copy (16 bytes per lane per iteration, grid-stride): the
+// reference point for hardware byte counters.
+__global__ __launch_bounds__(256) void copy16_kernel(uint64_t src_addr, uint64_t dst_addr, uint64_t n16) {
+    const gu4 *src = reinterpret_cast<const gu4 *>(src_addr);
+    gu4 *dst = reinterpret_cast<gu4 *>(dst_addr);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+}
+}  // namespace
+
+hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(copy16_kernel, dim3(grid), dim3(256), 0, s, (uint64_t)(uintptr_t)src,
+                       (uint64_t)(uintptr_t)dst, n16);
+    return hipGetLastError();
+}
+
 }  // namespace mtgp

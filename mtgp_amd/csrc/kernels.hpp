@@ -26,4 +26,7 @@ const void *jump_kernel_fn(int bits, int cls);
 hipError_t launch_jumps(int bits, int cls, const JumpItem *items, int n_items, const DevParam *params,
                         uint32_t *queue, int grid, size_t dyn_lds, hipStream_t s);
 
+// Copies n16 16-byte units src -> dst (device memory, 16-byte aligned).
+hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, int grid, hipStream_t s);
+
 }  // namespace mtgp

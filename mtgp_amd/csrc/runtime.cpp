@@ -97,6 +97,11 @@ Context &context() {
         if (v > 0)
             c->default_bpc = v;
     }
+    if (const char *e = getenv("MTGP_SEGMENT_ROUNDS")) {
+        const double v = atof(e);
+        if (v >= 0.25 && v <= 64)
+            c->segment_rounds = v;
+    }
     if (const char *e = getenv("MTGP_LAYOUT")) {
         const int lay = atoi(e);
         if (lay >= 0 && lay < kLayouts)
@@ -385,13 +390,12 @@ int append_jump_masks(const std::vector<const std::vector<uint32_t> *> &polys, s
 }
 
 uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, int slots) {
-    // All segments of a launch run at once when there are no more of them
-    // than resident workgroups, and the launch then takes as long as one
-    // segment (plus its start-state jump).  So: as many equal segments as
-    // there are workgroup slots, leaving room for one partial segment per
-    // request.
-    (void)c;
-    const uint64_t usable = (uint64_t)std::max(1, slots - n_requests);
+    // Segments of one launch run side by side, so a launch takes about
+    // (its work / resident slots) plus the tail of the last round.  About
+    // `segment_rounds` equal segments per slot, less room for the one partial
+    // segment each request ends with.
+    const double want = c.segment_rounds * (double)slots - (double)n_requests;
+    const uint64_t usable = (uint64_t)std::max(1.0, want);
     uint64_t s = (total_numbers + usable - 1) / usable;
     s = std::max<uint64_t>(s, 4096);
     return (s + 255) & ~255ull;
@@ -1010,6 +1014,17 @@ int mtgp_memcpy_htod(void *dst, const void *src, size_t bytes) {
 int mtgp_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
     API_BEGIN
     hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    API_BEGIN
+    Context &c = context();
+    if (((uintptr_t)dst | (uintptr_t)src | bytes) & 15)
+        throw Error("mtgp_device_copy needs 16-byte aligned pointers and size");
+    if (bytes)
+        hip_check(launch_copy16(src, dst, bytes / 16, c.cus * 8, (hipStream_t)stream), "copy launch");
     return 0;
     API_END(-1)
 }

@@ -39,6 +39,11 @@ struct Context {
     // Resident workgroups per CU used by default (measured best for the
     // generation kernels: 16 of the 32 the occupancy limit allows).
     int default_bpc = 16;
+    // Automatic segmentation cuts the work of a plan into about this many
+    // segments per resident workgroup slot: more rounds balance the tail
+    // better, each extra segment costs one jump-ahead (1.5 measured best for
+    // uneven MTGP32-11213 batches at 16 workgroups per CU).
+    double segment_rounds = 1.5;
     struct Custom {
         DevParam rec;
         unsigned char sha1[20];
