@@ -54,6 +54,74 @@ def batched_workload(n_gen=128):
     return seeds, lengths
 
 
+def serving_workload(n_requests=6000, rate=10000.0, n_gen=256, log2_mean=16.0, log2_sd=2.5, log2_min=10,
+                     log2_max=26, jump_frac=0.1, seed=1):
+    """The serving configuration: Poisson arrivals at `rate` per second over
+    n_gen device generators -- half MTGP32-11213, a quarter MTGP32-23209, a
+    quarter MTGP64-23209 (sets 0, 1, ... of each table), seeded with
+    splitmix64 started at 2024 -- each request asking for uint32, float in
+    [1,2) or double in [0,1) with equal probability (doubles from the MTGP64
+    generators, the others from the MTGP32 ones, uniformly), 2^x numbers
+    with x ~ Normal(log2_mean, log2_sd) clipped to [log2_min, log2_max],
+    and with probability jump_frac a skip of 2^u numbers, u ~ U[10, 30],
+    before its output.  Returns (specs, seeds, trace)."""
+    n32a, n32b = n_gen // 2, n_gen // 4
+    n64 = n_gen - n32a - n32b
+    specs = ([(32, 11213, i % 128) for i in range(n32a)] + [(32, 23209, i % 128) for i in range(n32b)]
+             + [(64, 23209, i % 128) for i in range(n64)])
+    seeds = splitmix64(2024, n_gen)
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1e6 / rate, n_requests))
+    dtype = rng.integers(0, 3, n_requests)
+    gen = np.where(dtype < 2, rng.integers(0, n32a + n32b, n_requests), n32a + n32b + rng.integers(0, n64, n_requests))
+    kind = np.array([0, 1, 2])[dtype]
+    x = np.clip(rng.normal(log2_mean, log2_sd, n_requests), log2_min, log2_max)
+    count = np.round(2.0 ** x).astype(np.uint64)
+    skip = np.where(rng.random(n_requests) < jump_frac, np.floor(2.0 ** rng.uniform(10, 30, n_requests)), 0)
+    trace = {"t_us": t, "gen": gen.astype(np.int32), "kind": kind.astype(np.int32), "count": count,
+             "skip": skip.astype(np.uint64)}
+    return specs, seeds, trace
+
+
+def run_serving(M, n_requests, rate, **server_args):
+    """Replays one serving trace through the batched server and through the
+    plain path (one launch per request, arrival order), from the same
+    initial states; returns latency percentiles, throughput and whether the
+    two produced identical outputs (digest of every request, in a second
+    replay of each)."""
+    specs, seeds, tr = serving_workload(n_requests=n_requests, rate=rate)
+    out = {}
+    digests = {}
+    for mode in ("batched", "plain"):
+        for check in (False, True):
+            gs = M.GenSet(specs)
+            gs.init_seed(seeds)
+            srv = M.Server(gs, plain=(mode == "plain"), **server_args)
+            queue = np.zeros(len(tr["t_us"]))
+            lat, dg, wall = srv.replay(tr["t_us"], tr["gen"], tr["kind"], tr["count"], tr["skip"], digests=check,
+                                       queue=queue)
+            st = srv.stats()
+            srv.close()
+            if check:
+                digests[mode] = dg
+                continue
+            nbytes = float(sum(int(c) * (8 if k == 2 else 4) for c, k in zip(tr["count"], tr["kind"])))
+            out[mode] = {"p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 4),
+                         "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 4),
+                         "mean_ms": round(float(lat.mean()) / 1e3, 4),
+                         "queue_p50_ms": round(float(np.percentile(queue, 50)) / 1e3, 4),
+                         "queue_p99_ms": round(float(np.percentile(queue, 99)) / 1e3, 4),
+                         "gbs": round(nbytes / (wall * 1e-6) / 1e9, 3), "wall_s": round(wall * 1e-6, 4),
+                         "batches": int(st["batches"]), "max_batch_requests": int(st["max_batch_requests"]),
+                         "plan_us_per_batch": round(st["plan_us"] / max(1, st["batches"]), 1)}
+    out["bit_exact_equal"] = bool((digests["batched"] == digests["plain"]).all())
+    out["requests"] = n_requests
+    out["rate_per_s"] = rate
+    out["offered_gbs"] = round(float(sum(int(c) * (8 if k == 2 else 4) for c, k in zip(tr["count"], tr["kind"])))
+                               / (tr["t_us"][-1] * 1e-6) / 1e9, 3)
+    return out
+
+
 def run_gpu(args):
     import mtgp_amd as M
     seeds, lengths = batched_workload()

@@ -127,10 +127,12 @@ typedef struct mtgp_request {
     void *out;
 } mtgp_request_t;
 
-/* Batched generation.  A plan fixes the requests, splits each generator's
- * stream into segments of about `segment` numbers (0 = chosen from the
- * size of the batch) whose start states are reached by device jump-ahead,
- * and stores the work list in device memory.  mtgp_plan_launch enqueues
+/* Batched generation.  A plan fixes the requests and cuts each generator's
+ * stream positions on a grid of `segment` numbers (0 = chosen from the size
+ * of the batch) into cells: one work item per cell, which starts from a
+ * device jump-ahead (or the generator's state) and runs the pieces of the
+ * consecutive requests that fall into the cell, stepping over short skips.
+ * The work list lives in device memory.  mtgp_plan_launch enqueues
  * one persistent kernel launch per (word size, state size class) on
  * `stream`; it can be called again (the same numbers are produced again,
  * from the same states) and captured in a hipGraph.  mtgp_plan_commit
@@ -148,10 +150,93 @@ int mtgp_plan_configure(mtgp_plan_t *plan, int layout, int blocks_per_cu);
 /* Enqueues one part of a plan: 0 = the start-state jump launches, 1 = the
  * generation launches (part 1 reads what part 0 wrote: order them). */
 int mtgp_plan_launch_part(mtgp_plan_t *plan, int part, void *stream);
+/* Records the plan's launches (part 0 then part 1) in a hipGraph on
+ * `stream` (not the null stream) and instantiates it; the graph contains
+ * kernel nodes only (the work-queue counters reset themselves).
+ * mtgp_plan_launch_graph replays it on any stream.  Configuring the plan
+ * drops the graph. */
+int mtgp_plan_capture(mtgp_plan_t *plan, void *stream);
+int mtgp_plan_launch_graph(mtgp_plan_t *plan, void *stream);
+/* Autotuner: runs the start-state jumps once, then times the generation
+ * launches `reps` times for each step layout and resident workgroups per CU
+ * (4..32, capped by occupancy), configures the fastest and makes it the
+ * default for later plans of the same word size and state-size class.
+ * Writes the choice and its time per generation pass.  The outputs are
+ * written (identically) by every trial. */
+int mtgp_plan_autotune(mtgp_plan_t *plan, void *stream, int reps, int *layout, int *blocks_per_cu,
+                       float *best_ms);
 /* Work items, kernel launches, and host planning time of a plan. */
 int mtgp_plan_stats(const mtgp_plan_t *plan, int *items, int *launches, double *plan_ms);
 /* plan + launch + commit + synchronize. */
 int mtgp_genset_generate(mtgp_genset_t *set, const mtgp_request_t *reqs, int nreq, void *stream);
+
+/* ---------------------------------------------------------------- serving
+ * A request server over a generator set (no counterpart in the reference,
+ * whose samples launch one kernel per call).  Requests are submitted at any
+ * time from any thread; a scheduler thread admits them in arrival order
+ * while their output fits the output-memory budget (one device arena),
+ * coalesces everything admitted into batches -- one plan per state-size
+ * class -- and launches each batch as soon as it is planned, on one of
+ * several streams.  A batch waits (device-side event) only for earlier
+ * in-flight batches that advance one of its generators, so new work
+ * overlaps work already on the GPU.  Per-generator order is the submission
+ * order: every result is exactly what the same requests made one at a time
+ * would produce.  While a server runs it owns the set's states.
+ * flags: MTGP_SERVER_PLAIN = one request per launch on a single stream, in
+ * arrival order (the baseline); MTGP_SERVER_FUSED_JUMPS, below. */
+typedef struct mtgp_server mtgp_server_t;
+typedef struct mtgp_server_config {
+    size_t arena_bytes;   /* output-memory budget (0: 4 GiB) */
+    uint64_t cell;        /* numbers per work item of a large request (power of two; 0: 2^17) */
+    int cached_cells;     /* jump polynomials kept per parameter set: x^(k cell), k <= this (0: 512) */
+    int streams;          /* device streams (0: 4) */
+    int max_inflight;     /* batches on the device at once (0: 8) */
+    int max_batch;        /* requests per batch at most (0: 256) */
+    int flags;
+} mtgp_server_config_t;
+#define MTGP_SERVER_PLAIN 1
+/* Work items jump to their start state themselves (one launch per batch)
+ * instead of a batched jump launch first. */
+#define MTGP_SERVER_FUSED_JUMPS 2
+typedef struct mtgp_completion {
+    int64_t ticket;
+    void *out;          /* device address of the output, in the arena */
+    uint64_t bytes;
+    double submit_us;   /* server clock (microseconds since creation) */
+    double launch_us;
+    double done_us;     /* when the scheduler saw the batch finish */
+} mtgp_completion_t;
+typedef struct mtgp_server_stats {
+    int64_t submitted, completed, batches, launches;
+    double numbers;           /* numbers produced */
+    double plan_us;           /* host planning time, total */
+    int64_t max_batch_requests;
+} mtgp_server_stats_t;
+mtgp_server_t *mtgp_server_create(mtgp_genset_t *set, const mtgp_server_config_t *config);
+/* Drains, stops the scheduler and frees the arena (outputs included). */
+int mtgp_server_destroy(mtgp_server_t *server);
+/* Enqueues `count` numbers of `kind` from generator `gen` after skipping
+ * `skip` numbers; returns a ticket >= 0, or -1 (mtgp_last_error()). */
+int64_t mtgp_server_submit(mtgp_server_t *server, int gen, int kind, uint64_t count, uint64_t skip);
+/* Up to `max` completions, oldest first; waits up to wait_ms for one
+ * (0: no wait).  Returns how many were written. */
+int mtgp_server_poll(mtgp_server_t *server, mtgp_completion_t *out, int max, int wait_ms);
+/* Gives a completed request's output space back to the budget. */
+int mtgp_server_release(mtgp_server_t *server, int64_t ticket);
+/* Waits until every request submitted so far has completed. */
+int mtgp_server_drain(mtgp_server_t *server);
+int mtgp_server_stats(mtgp_server_t *server, mtgp_server_stats_t *stats);
+/* Replays a trace: entry i is submitted `t_us[i]` microseconds after the
+ * start, from this thread; a second thread polls the completions and
+ * releases them.  Writes each request's latency (completion - submission,
+ * microseconds), optionally its queueing delay (launch - submission), and
+ * with `digests` != NULL a 64-bit digest of each output (copied to the
+ * host: slower, for checking).  Returns 0 or -1. */
+int mtgp_server_replay(mtgp_server_t *server, int n, const double *t_us, const int32_t *gen,
+                       const int32_t *kind, const uint64_t *count, const uint64_t *skip,
+                       double *latency_us, double *queue_us, uint64_t *digests, double *wall_us);
+/* The 64-bit digest mtgp_server_replay computes (of any host bytes). */
+uint64_t mtgp_digest64(const void *data, size_t bytes);
 
 #if defined(__cplusplus)
 }

@@ -138,7 +138,40 @@ _sig("mtgp_plan_destroy", None, vp)
 _sig("mtgp_plan_configure", i32, vp, i32, i32)
 _sig("mtgp_plan_launch_part", i32, vp, i32, vp)
 _sig("mtgp_plan_stats", i32, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double))
+_sig("mtgp_plan_capture", i32, vp, vp)
+_sig("mtgp_plan_launch_graph", i32, vp, vp)
+_sig("mtgp_plan_autotune", i32, vp, vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float))
 _sig("mtgp_genset_generate", i32, vp, C.POINTER(Request), i32, vp)
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("arena_bytes", C.c_size_t), ("cell", C.c_uint64), ("cached_cells", C.c_int),
+                ("streams", C.c_int), ("max_inflight", C.c_int), ("max_batch", C.c_int), ("flags", C.c_int)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("ticket", C.c_int64), ("out", C.c_void_p), ("bytes", C.c_uint64), ("submit_us", C.c_double),
+                ("launch_us", C.c_double), ("done_us", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("batches", C.c_int64),
+                ("launches", C.c_int64), ("numbers", C.c_double), ("plan_us", C.c_double),
+                ("max_batch_requests", C.c_int64)]
+
+
+SERVER_PLAIN = 1
+SERVER_FUSED_JUMPS = 2
+_dp, _i32p, _u64p = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)
+_sig("mtgp_server_create", vp, vp, C.POINTER(ServerConfig))
+_sig("mtgp_server_destroy", i32, vp)
+_sig("mtgp_server_submit", C.c_int64, vp, i32, i32, u64, u64)
+_sig("mtgp_server_poll", i32, vp, C.POINTER(Completion), i32, i32)
+_sig("mtgp_server_release", i32, vp, C.c_int64)
+_sig("mtgp_server_drain", i32, vp)
+_sig("mtgp_server_stats", i32, vp, C.POINTER(ServerStats))
+_sig("mtgp_server_replay", i32, vp, i32, _dp, _i32p, _i32p, _u64p, _u64p, _dp, _dp, _u64p, _dp)
+_sig("mtgp_digest64", C.c_uint64, vp, sz)
 
 lib = _lib
 
@@ -207,6 +240,12 @@ class DeviceBuffer:
 
 def synchronize():
     _check(_lib.mtgp_synchronize(), "mtgp_synchronize")
+
+
+def memcpy_dtoh(host, device_address):
+    """Copies host.nbytes bytes from a device address into a host array."""
+    _check(_lib.mtgp_memcpy_dtoh(host.ctypes.data, device_address, host.nbytes), "memcpy dtoh")
+    return host
 
 
 def device_copy(dst, src, nbytes, stream=None):
@@ -433,6 +472,21 @@ class Plan:
     def configure(self, layout=-1, blocks_per_cu=-1):
         _check(_lib.mtgp_plan_configure(self.h, layout, blocks_per_cu), "plan_configure")
 
+    def capture(self, stream):
+        """Records the plan's launches in a hipGraph (stream: a Stream)."""
+        _check(_lib.mtgp_plan_capture(self.h, stream.handle), "plan_capture")
+
+    def launch_graph(self, stream=None):
+        _check(_lib.mtgp_plan_launch_graph(self.h, stream.handle if stream else None), "plan_launch_graph")
+
+    def autotune(self, stream=None, reps=2):
+        """Times the generation launch per (layout, workgroups per CU) and
+        keeps the fastest; returns (layout, blocks_per_cu, ms)."""
+        lay, bpc, ms = C.c_int(), C.c_int(), C.c_float()
+        _check(_lib.mtgp_plan_autotune(self.h, stream.handle if stream else None, reps, C.byref(lay), C.byref(bpc),
+                                       C.byref(ms)), "plan_autotune")
+        return lay.value, bpc.value, ms.value
+
     def stats(self):
         it, ln, ms = C.c_int(), C.c_int(), C.c_double()
         _lib.mtgp_plan_stats(self.h, C.byref(it), C.byref(ln), C.byref(ms))
@@ -442,3 +496,81 @@ class Plan:
         if getattr(self, "h", None):
             _lib.mtgp_plan_destroy(self.h)
             self.h = None
+
+
+# ---------------------------------------------------------------- serving
+def digest64(a):
+    """The 64-bit digest the server's replay computes, of a host array."""
+    a = np.ascontiguousarray(a)
+    return int(_lib.mtgp_digest64(a.ctypes.data, a.nbytes))
+
+
+class Server:
+    """Continuous-batching request server over a GenSet (mtgp_server_t)."""
+
+    def __init__(self, genset, arena_bytes=0, cell=0, cached_cells=0, streams=0, max_inflight=0, max_batch=0,
+                 plain=False, fused_jumps=False):
+        flags = (SERVER_PLAIN if plain else 0) | (SERVER_FUSED_JUMPS if fused_jumps else 0)
+        cfg = ServerConfig(arena_bytes, cell, cached_cells, streams, max_inflight, max_batch, flags)
+        self.genset = genset
+        self.h = _lib.mtgp_server_create(genset.h, C.byref(cfg))
+        if not self.h:
+            raise MTGPError("server_create: " + last_error())
+
+    def submit(self, gen, kind, count, skip=0):
+        t = _lib.mtgp_server_submit(self.h, gen, KINDS.get(kind, kind), count, skip)
+        if t < 0:
+            raise MTGPError("server_submit: " + last_error())
+        return t
+
+    def poll(self, max_n=256, wait_ms=0):
+        arr = (Completion * max_n)()
+        k = _lib.mtgp_server_poll(self.h, arr, max_n, wait_ms)
+        if k < 0:
+            raise MTGPError("server_poll: " + last_error())
+        return [dict(ticket=c.ticket, out=c.out, bytes=c.bytes, submit_us=c.submit_us, launch_us=c.launch_us,
+                     done_us=c.done_us) for c in arr[:k]]
+
+    def release(self, ticket):
+        _check(_lib.mtgp_server_release(self.h, ticket), "server_release")
+
+    def drain(self):
+        _check(_lib.mtgp_server_drain(self.h), "server_drain")
+
+    def stats(self):
+        st = ServerStats()
+        _check(_lib.mtgp_server_stats(self.h, C.byref(st)), "server_stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def replay(self, t_us, gen, kind, count, skip, digests=False, queue=None):
+        """Submits request i at t_us[i] microseconds; returns (latency_us,
+        digests or None, wall_us).  `queue`: optional float64 array that
+        receives each request's launch - submission delay."""
+        n = len(t_us)
+        t = np.ascontiguousarray(t_us, np.float64)
+        g = np.ascontiguousarray(gen, np.int32)
+        k = np.ascontiguousarray(kind, np.int32)
+        c = np.ascontiguousarray(count, np.uint64)
+        sk = np.ascontiguousarray(skip, np.uint64)
+        lat = np.zeros(n, np.float64)
+        dg = np.zeros(n, np.uint64) if digests else None
+        wall = C.c_double(0)
+        _check(_lib.mtgp_server_replay(self.h, n, t.ctypes.data_as(_dp), g.ctypes.data_as(_i32p),
+                                       k.ctypes.data_as(_i32p), c.ctypes.data_as(_u64p), sk.ctypes.data_as(_u64p),
+                                       lat.ctypes.data_as(_dp),
+                                       queue.ctypes.data_as(_dp) if queue is not None else None,
+                                       dg.ctypes.data_as(_u64p) if digests else None,
+                                       C.byref(wall)), "server_replay")
+        return lat, dg, wall.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            h, self.h = self.h, None
+            _check(_lib.mtgp_server_destroy(h), "server_destroy")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

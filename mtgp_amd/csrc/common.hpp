@@ -6,8 +6,21 @@
 
 namespace mtgp {
 
+// One piece of a work item's stream: `count` numbers of `kind` written to
+// `out`, or, with out == 0, `count` numbers stepped over without output.
+struct alignas(16) Segment {
+    uint64_t out;
+    uint64_t count;
+    int32_t kind;
+    int32_t pad0;
+    uint64_t pad1;
+};
+static_assert(sizeof(Segment) == 32, "Segment layout");
+
 // One unit of device work: continue (or first jump, then continue) the
-// stream of one generator for `count` numbers of one output kind.
+// stream of one generator for `count` numbers of one output kind, or, with
+// nsegs > 0, through the segments segs[0, nsegs) in order (consecutive
+// requests of one generator run by one wave with no jump between them).
 //
 //  - state_in  : canonical window (N words, oldest first; 64-bit words for
 //                MTGP64) the stream continues from.
@@ -28,13 +41,16 @@ struct alignas(16) Item {
     int32_t parts;       // >1: the start window is the XOR of `parts`
                          // partial windows, part_bytes apart, at state_in
     uint64_t part_bytes;
+    uint64_t segs;       // const Segment * (used when nsegs > 0)
+    int32_t nsegs;
+    int32_t pad0;
 };
-static_assert(sizeof(Item) == 64, "Item layout");
+static_assert(sizeof(Item) == 80, "Item layout");
 
 // Batched jump: nb (<= kJumpBatch) jumps of one base window, each written
 // to states_out + b * state_bytes.  The coefficients come pre-arranged by
-// chunk and residue: masks[(c * 64 + rho) * kJumpBatch + b] holds, in bit q,
-// the coefficient of x^(c * kJumpChunk + 64 q + rho) of jump b.
+// chunk and residue: the 64-bit word masks[c * 64 + rho] holds, in bit
+// 16 b + q, the coefficient of x^(c * kJumpChunk + 64 q + rho) of jump b.
 constexpr int kJumpBatch = 4;
 constexpr int kJumpSteps = 4;                  // 256-word steps per chunk
 constexpr int kJumpChunk = kJumpSteps * 256;   // positions per chunk

@@ -163,10 +163,10 @@ __device__ __forceinline__ void step32s(uint32_t *buf, const uint32_t *tab, cons
     }
 }
 
-// Copies the window [b, b + n) to [0, n) through registers (the ranges may
-// overlap).
+// Copies the window [b, b + n) to [to, to + n) through registers (the
+// ranges may overlap).
 template <int NJ>
-__device__ __forceinline__ void relocate32(uint32_t *buf, int b, int n) {
+__device__ __forceinline__ void relocate32(uint32_t *buf, int b, int n, int to = 0) {
     const int l = threadIdx.x;
     uint32_t v[NJ];
 #pragma unroll
@@ -176,7 +176,7 @@ __device__ __forceinline__ void relocate32(uint32_t *buf, int b, int n) {
 #pragma unroll
     for (int m = 0; m < NJ; m++)
         if (l + kWave * m < n)
-            buf[l + kWave * m] = v[m];
+            buf[to + l + kWave * m] = v[m];
     wave_sync();
 }
 
@@ -254,39 +254,83 @@ __device__ void jump32(uint32_t *buf, const uint32_t *tab, const P32 &p, const g
     wave_sync();
 }
 
-template <int KIND, int NJ, int NBUF, int LAYOUT>
-__device__ void generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int o, int h,
-                           gu32 *__restrict__ out, uint64_t count, gu32 *__restrict__ state_out) {
-    const int l = threadIdx.x;
-    // Head (layout 0): h < 4 words, one per lane, so that the main loop
-    // stores are 16-byte aligned; the window base becomes o + h == 0 mod 4.
-    // (Layout 1 takes its head below as a partial step.)
-    if (LAYOUT == 0 && h > 0) {
-        if (l < h) {
-            const int q = o + l;
-            const uint32_t r = rec32(p, tab, buf[q], buf[q + 1], buf[q + p.pos]);
-            buf[q + p.n] = r;
-            out[l] = out32<KIND>(tab, r, buf[q + p.pos - 1]);
-        }
-        wave_sync();
+// Numbers before the first aligned store of an output at `out`: to 16 bytes
+// (layout 0) or to a 128-byte line (layout 1).
+template <int LAYOUT>
+__device__ __forceinline__ uint32_t head32(uint64_t out, uint64_t count) {
+    const uint32_t h = LAYOUT == 0 ? ((16u - (uint32_t)(out & 15)) & 15) >> 2
+                                   : ((128u - (uint32_t)(out & 127)) & 127) >> 2;
+    return (uint64_t)h > count ? (uint32_t)count : h;
+}
+
+// Window base for a segment with head h: layout 0 needs (b + h) % 4 == 0;
+// both need room for one more step.  Moves the window when either fails.
+template <int NJ, int NBUF, int LAYOUT>
+__device__ __forceinline__ int place32(uint32_t *buf, const P32 &p, int b, uint32_t h) {
+    const int to = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
+    const bool misaligned = LAYOUT == 0 && (((uint32_t)b + h) & 3u) != 0;
+    if (misaligned || b + p.n + kStep > NBUF) {
+        relocate32<NJ>(buf, b, p.n, to);
+        return to;
     }
-    // count - h numbers: whole steps, then one partial step.  (Written
-    // without a select on the 64-bit remainder.)
-    int b = o + (LAYOUT == 0 ? h : 0);
+    return b;
+}
+
+// Steps the window over `count` numbers without output; returns the new base.
+template <int NJ, int NBUF>
+__device__ int advance32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b, uint64_t count) {
+    for (uint64_t s = count / kStep; s > 0; s--) {
+        if (b + p.n + kStep > NBUF) {
+            relocate32<NJ>(buf, b, p.n);
+            b = 0;
+        }
+        step32s<0, false>(buf, tab, p, b, nullptr, 0);
+        wave_sync();
+        b += kStep;
+    }
+    const uint32_t m = (uint32_t)(count % kStep);
+    if (m) {
+        if (b + p.n + kStep > NBUF) {
+            relocate32<NJ>(buf, b, p.n);
+            b = 0;
+        }
+        step32s<0, false>(buf, tab, p, b, nullptr, 0);
+        wave_sync();
+        b += (int)m;
+    }
+    return b;
+}
+
+// Writes `count` numbers of KIND to out from the window at base b (placed
+// by place32 for head h); returns the window base after the last number.
+template <int KIND, int NJ, int NBUF, int LAYOUT>
+__device__ int generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b, uint32_t h,
+                          gu32 *__restrict__ out, uint64_t count) {
+    const int l = threadIdx.x;
     gu32 *dst = out;
     uint64_t rest = count;
-    if (LAYOUT == 0) {
-        dst += h;
-        rest -= (uint64_t)h;
-    } else if (h > 0) {
-        // Layout 1: h < 32 numbers so that every later 256-byte store starts
-        // on a 128-byte line (no partially written lines).
-        step32s<KIND, true>(buf, tab, p, b, dst, (uint32_t)h);
+    if (h > 0) {
+        if (LAYOUT == 0) {
+            // h < 4 words, one per lane, so that the main loop's stores are
+            // 16-byte aligned (the window base becomes b + h == 0 mod 4).
+            if ((uint32_t)l < h) {
+                const int q = b + l;
+                const uint32_t r = rec32(p, tab, buf[q], buf[q + 1], buf[q + p.pos]);
+                buf[q + p.n] = r;
+                out[l] = out32<KIND>(tab, r, buf[q + p.pos - 1]);
+            }
+        } else {
+            // h < 32 numbers so that every later 256-byte store starts on a
+            // 128-byte line (no partially written lines).
+            step32s<KIND, true>(buf, tab, p, b, dst, h);
+        }
         wave_sync();
-        b += h;
+        b += (int)h;
         dst += h;
         rest -= (uint64_t)h;
     }
+    // Whole steps, then one partial step.  (Written without a select on the
+    // 64-bit remainder.)
     for (uint64_t s = rest / kStep; s > 0; s--) {
         if (b + p.n + kStep > NBUF) {
             relocate32<NJ>(buf, b, p.n);
@@ -313,9 +357,27 @@ __device__ void generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int
         wave_sync();
         b += (int)m;
     }
-    if (state_out)
-        for (int j = l; j < p.n; j += kWave)
-            state_out[j] = buf[b + j];
+    return b;
+}
+
+// One segment (Segment, common.hpp) from window base b; returns the new base.
+template <int NJ, int NBUF, int LAYOUT>
+__device__ int segment32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b, uint64_t out, uint64_t count,
+                         int kind) {
+    if (count == 0)
+        return b;
+    if (out == 0)
+        return advance32<NJ, NBUF>(buf, tab, p, b, count);
+    const uint32_t h = head32<LAYOUT>(out, count);
+    b = place32<NJ, NBUF, LAYOUT>(buf, p, b, h);
+    gu32 *dst = reinterpret_cast<gu32 *>(out);
+    switch (kind) {
+    case 0: return generate32<0, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
+    case 1: return generate32<1, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
+    case 2: return generate32<2, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
+    case 3: return generate32<3, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
+    default: return generate32<4, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
+    }
 }
 
 template <int NMAX, int K, int LAYOUT>
@@ -339,17 +401,8 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
         p.mask = D.mask_lo;
         if (l < 48)
             tab[l] = D.tbl[l];  // tbl, tmp, flt are contiguous
-        const uint64_t count = I.count;
-        gu32 *out = reinterpret_cast<gu32 *>(I.out);
         gu32 *state_out = reinterpret_cast<gu32 *>(I.state_out);
         const gu32 *state_in = reinterpret_cast<const gu32 *>(I.state_in);
-        // Numbers before the first aligned store: to 16 bytes (layout 0) or
-        // to a 128-byte line (layout 1).
-        uint32_t h = LAYOUT == 0 ? (uint32_t)((16u - (uint32_t)(I.out & 15)) & 15) >> 2
-                                 : (uint32_t)((128u - (uint32_t)(I.out & 127)) & 127) >> 2;
-        if ((uint64_t)h > count)
-            h = (uint32_t)count;
-        const int o = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
         wave_sync();
         if (I.poly) {
             uint32_t acc[NJ];
@@ -358,7 +411,7 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
 #pragma unroll
             for (int m = 0; m < NJ; m++)
                 if (l + kWave * m < p.n)
-                    buf[o + l + kWave * m] = acc[m];
+                    buf[l + kWave * m] = acc[m];
         } else {
             const int parts = I.parts > 1 ? I.parts : 1;
             const uint64_t pstride = I.part_bytes / 4;
@@ -366,17 +419,21 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
                 uint32_t v = state_in[j];
                 for (int q = 1; q < parts; q++)
                     v ^= state_in[(uint64_t)q * pstride + (uint64_t)j];
-                buf[o + j] = v;
+                buf[j] = v;
             }
         }
         wave_sync();
-        switch (I.kind) {
-        case 0: generate32<0, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
-        case 1: generate32<1, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
-        case 2: generate32<2, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
-        case 3: generate32<3, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
-        default: generate32<4, NJ, NBUF, LAYOUT>(buf, tab, p, o, (int)h, out, count, state_out); break;
+        int b = 0;
+        if (I.nsegs > 0) {
+            const GMEM Segment *segs = reinterpret_cast<const GMEM Segment *>(I.segs);
+            for (int si = 0; si < I.nsegs; si++)
+                b = segment32<NJ, NBUF, LAYOUT>(buf, tab, p, b, segs[si].out, segs[si].count, segs[si].kind);
+        } else {
+            b = segment32<NJ, NBUF, LAYOUT>(buf, tab, p, b, I.out, I.count, I.kind);
         }
+        if (state_out)
+            for (int j = l; j < p.n; j += kWave)
+                state_out[j] = buf[b + j];
         wave_sync();
         uint32_t next = 0;
         if (l == 0)
@@ -529,7 +586,7 @@ __device__ __forceinline__ void step64s(uint32_t *bh, uint32_t *bl, const uint32
 }
 
 template <int NJ>
-__device__ __forceinline__ void relocate64(uint32_t *bh, uint32_t *bl, int b, int n) {
+__device__ __forceinline__ void relocate64(uint32_t *bh, uint32_t *bl, int b, int n, int to = 0) {
     const int l = threadIdx.x;
     uint32_t vh[NJ], vl[NJ];
 #pragma unroll
@@ -542,8 +599,8 @@ __device__ __forceinline__ void relocate64(uint32_t *bh, uint32_t *bl, int b, in
 #pragma unroll
     for (int m = 0; m < NJ; m++)
         if (l + kWave * m < n) {
-            bh[l + kWave * m] = vh[m];
-            bl[l + kWave * m] = vl[m];
+            bh[to + l + kWave * m] = vh[m];
+            bl[to + l + kWave * m] = vl[m];
         }
     wave_sync();
 }
@@ -620,32 +677,70 @@ __device__ void jump64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P6
     wave_sync();
 }
 
-template <int KIND, int NJ, int NBUF, int LAYOUT>
-__device__ void generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int o, int h,
-                           gu32 *__restrict__ out, uint64_t count, gu64 *__restrict__ state_out) {
-    const int l = threadIdx.x;
-    if (LAYOUT == 0 && h > 0) {
-        if (l < h) {
-            const int q = o + l;
-            uint32_t rh, rl, oh, ol;
-            rec64(p, tab, bh[q], bl[q], bh[q + 1], bl[q + 1], bh[q + p.pos], bl[q + p.pos], rh, rl);
-            bh[q + p.n] = rh;
-            bl[q + p.n] = rl;
-            out64<KIND>(tab, rh, rl, bl[q + p.pos - 1], oh, ol);
-            reinterpret_cast<gu2 *>(out)[l] = v2u{ol, oh};
-        }
-        wave_sync();
+// As head32 for 8-byte outputs.
+template <int LAYOUT>
+__device__ __forceinline__ uint32_t head64(uint64_t out, uint64_t count) {
+    const uint32_t h = LAYOUT == 0 ? ((out & 15) ? 1u : 0u) : ((128u - (uint32_t)(out & 127)) & 127) >> 3;
+    return (uint64_t)h > count ? (uint32_t)count : h;
+}
+
+template <int NJ, int NBUF, int LAYOUT>
+__device__ __forceinline__ int place64(uint32_t *bh, uint32_t *bl, const P64 &p, int b, uint32_t h) {
+    const int to = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
+    const bool misaligned = LAYOUT == 0 && (((uint32_t)b + h) & 3u) != 0;
+    if (misaligned || b + p.n + kStep > NBUF) {
+        relocate64<NJ>(bh, bl, b, p.n, to);
+        return to;
     }
-    int b = o + (LAYOUT == 0 ? h : 0);
+    return b;
+}
+
+template <int NJ, int NBUF>
+__device__ int advance64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b, uint64_t count) {
+    for (uint64_t s = count / kStep; s > 0; s--) {
+        if (b + p.n + kStep > NBUF) {
+            relocate64<NJ>(bh, bl, b, p.n);
+            b = 0;
+        }
+        step64s<0, false>(bh, bl, tab, p, b, nullptr, 0);
+        wave_sync();
+        b += kStep;
+    }
+    const uint32_t m = (uint32_t)(count % kStep);
+    if (m) {
+        if (b + p.n + kStep > NBUF) {
+            relocate64<NJ>(bh, bl, b, p.n);
+            b = 0;
+        }
+        step64s<0, false>(bh, bl, tab, p, b, nullptr, 0);
+        wave_sync();
+        b += (int)m;
+    }
+    return b;
+}
+
+template <int KIND, int NJ, int NBUF, int LAYOUT>
+__device__ int generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b, uint32_t h,
+                          gu32 *__restrict__ out, uint64_t count) {
+    const int l = threadIdx.x;
     gu32 *dst = out;
     uint64_t rest = count;
-    if (LAYOUT == 0) {
-        dst += 2 * h;
-        rest -= (uint64_t)h;
-    } else if (h > 0) {
-        step64s<KIND, true>(bh, bl, tab, p, b, dst, (uint32_t)h);
+    if (h > 0) {
+        if (LAYOUT == 0) {
+            if ((uint32_t)l < h) {
+                const int q = b + l;
+                uint32_t rh, rl, oh, ol;
+                rec64(p, tab, bh[q], bl[q], bh[q + 1], bl[q + 1], bh[q + p.pos], bl[q + p.pos], rh, rl);
+                bh[q + p.n] = rh;
+                bl[q + p.n] = rl;
+                out64<KIND>(tab, rh, rl, bl[q + p.pos - 1], oh, ol);
+                reinterpret_cast<gu2 *>(out)[l] = v2u{ol, oh};
+            }
+        } else {
+            step64s<KIND, true>(bh, bl, tab, p, b, dst, h);
+        }
         wave_sync();
-        b += h;
+        b += (int)h;
         dst += 2 * h;
         rest -= (uint64_t)h;
     }
@@ -675,9 +770,26 @@ __device__ void generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, cons
         wave_sync();
         b += (int)m;
     }
-    if (state_out)
-        for (int j = l; j < p.n; j += kWave)
-            state_out[j] = ((uint64_t)bh[b + j] << 32) | bl[b + j];
+    return b;
+}
+
+template <int NJ, int NBUF, int LAYOUT>
+__device__ int segment64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b, uint64_t out,
+                         uint64_t count, int kind) {
+    if (count == 0)
+        return b;
+    if (out == 0)
+        return advance64<NJ, NBUF>(bh, bl, tab, p, b, count);
+    const uint32_t h = head64<LAYOUT>(out, count);
+    b = place64<NJ, NBUF, LAYOUT>(bh, bl, p, b, h);
+    gu32 *dst = reinterpret_cast<gu32 *>(out);
+    switch (kind) {
+    case 0: return generate64<0, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
+    case 1: return generate64<1, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
+    case 2: return generate64<2, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
+    case 3: return generate64<3, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
+    default: return generate64<4, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
+    }
 }
 
 template <int NMAX, int K, int LAYOUT>
@@ -703,15 +815,8 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         p.mask_lo = D.mask_lo;
         if (l < 48)
             tab[l] = D.tbl[l];
-        const uint64_t count = I.count;
-        gu32 *out = reinterpret_cast<gu32 *>(I.out);
         gu64 *state_out = reinterpret_cast<gu64 *>(I.state_out);
         const gu64 *state_in = reinterpret_cast<const gu64 *>(I.state_in);
-        uint32_t h = LAYOUT == 0 ? ((I.out & 15) ? 1u : 0u)  // 8-byte outputs
-                                 : (uint32_t)((128u - (uint32_t)(I.out & 127)) & 127) >> 3;
-        if ((uint64_t)h > count)
-            h = (uint32_t)count;
-        const int o = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
         wave_sync();
         if (I.poly) {
             uint32_t ah[NJ], al[NJ];
@@ -720,8 +825,8 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
 #pragma unroll
             for (int m = 0; m < NJ; m++)
                 if (l + kWave * m < p.n) {
-                    bh[o + l + kWave * m] = ah[m];
-                    bl[o + l + kWave * m] = al[m];
+                    bh[l + kWave * m] = ah[m];
+                    bl[l + kWave * m] = al[m];
                 }
         } else {
             const int parts = I.parts > 1 ? I.parts : 1;
@@ -730,18 +835,22 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
                 uint64_t v = state_in[j];
                 for (int q = 1; q < parts; q++)
                     v ^= state_in[(uint64_t)q * pstride + (uint64_t)j];
-                bh[o + j] = (uint32_t)(v >> 32);
-                bl[o + j] = (uint32_t)v;
+                bh[j] = (uint32_t)(v >> 32);
+                bl[j] = (uint32_t)v;
             }
         }
         wave_sync();
-        switch (I.kind) {
-        case 0: generate64<0, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
-        case 1: generate64<1, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
-        case 2: generate64<2, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
-        case 3: generate64<3, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
-        default: generate64<4, NJ, NBUF, LAYOUT>(bh, bl, tab, p, o, (int)h, out, count, state_out); break;
+        int b = 0;
+        if (I.nsegs > 0) {
+            const GMEM Segment *segs = reinterpret_cast<const GMEM Segment *>(I.segs);
+            for (int si = 0; si < I.nsegs; si++)
+                b = segment64<NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, segs[si].out, segs[si].count, segs[si].kind);
+        } else {
+            b = segment64<NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, I.out, I.count, I.kind);
         }
+        if (state_out)
+            for (int j = l; j < p.n; j += kWave)
+                state_out[j] = ((uint64_t)bh[b + j] << 32) | bl[b + j];
         wave_sync();
         uint32_t next = 0;
         if (l == 0)

@@ -8,12 +8,15 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -97,6 +100,9 @@ Context &context() {
         if (v > 0)
             c->default_bpc = v;
     }
+    for (int b = 0; b < 2; b++)
+        for (int cls = 0; cls < 3; cls++)
+            c->bpc[b][cls] = c->default_bpc;
     if (const char *e = getenv("MTGP_SEGMENT_ROUNDS")) {
         const double v = atof(e);
         if (v >= 0.25 && v <= 64)
@@ -332,61 +338,179 @@ namespace {
 
 int elem_bytes(int bits) { return bits / 8; }
 
-// Parallel map over [0, n) on up to 16 host threads.
+// Persistent pool of up to 15 worker threads (plus the caller) for
+// parallel_for; a call hands out indices from a shared counter.  Calls from
+// several threads at once are serialised; a call from inside a task runs
+// inline.
+thread_local bool t_in_pool = false;
+
+class Pool {
+  public:
+    static Pool &get() {
+        static Pool p;
+        return p;
+    }
+    void run(int n, const std::function<void(int)> &f) {
+        std::lock_guard<std::mutex> call(call_mu_);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &f;
+            n_ = n;
+            next_ = 0;
+            active_ = (int)workers_.size();
+            err_.clear();
+            gen_++;
+        }
+        cv_.notify_all();
+        t_in_pool = true;
+        work();
+        t_in_pool = false;
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return active_ == 0; });
+        fn_ = nullptr;
+        if (!err_.empty())
+            throw Error(err_);
+    }
+
+  private:
+    Pool() {
+        const int hw = (int)std::thread::hardware_concurrency();
+        const int nt = std::max(0, std::min(hw > 0 ? hw : 1, 16) - 1);
+        for (int t = 0; t < nt; t++)
+            workers_.emplace_back([this] { loop(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &w : workers_)
+            w.join();
+    }
+    void work() {
+        for (;;) {
+            const int i = next_++;
+            if (i >= n_)
+                return;
+            try {
+                (*fn_)(i);
+            } catch (const std::exception &e) {
+                std::lock_guard<std::mutex> lk(mu_);
+                err_ = e.what();
+                next_ = n_;
+            }
+        }
+    }
+    void loop() {
+        t_in_pool = true;
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_)
+                    return;
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--active_ == 0)
+                done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(int)> *fn_ = nullptr;
+    std::atomic<int> next_{0};
+    int n_ = 0, active_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    std::string err_;
+};
+
+// Parallel map over [0, n) (inline for n <= 1).
 template <class F>
 void parallel_for(int n, F f) {
-    int hw = (int)std::thread::hardware_concurrency();
-    int nt = std::max(1, std::min({n, hw > 0 ? hw : 1, 16}));
-    if (nt <= 1) {
+    if (n <= 1 || t_in_pool) {
         for (int i = 0; i < n; i++)
             f(i);
         return;
     }
-    std::atomic<int> next(0);
-    std::vector<std::string> errs((size_t)nt);
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; t++)
-        th.emplace_back([&, t] {
-            try {
-                for (int i; (i = next++) < n;)
-                    f(i);
-            } catch (const std::exception &e) {
-                errs[(size_t)t] = e.what();
-                next = n;
-            }
-        });
-    for (auto &x : th)
-        x.join();
-    for (auto &e : errs)
-        if (!e.empty())
-            throw Error(e);
+    const std::function<void(int)> fn = f;
+    Pool::get().run(n, fn);
 }
 
 }  // namespace
 
-int append_jump_masks(const std::vector<const std::vector<uint32_t> *> &polys, std::vector<uint64_t> &out) {
-    size_t ncoeff = 0;
-    for (const auto *w : polys)
-        ncoeff = std::max(ncoeff, 32 * w->size());
+std::vector<uint16_t> jump_mask16(const std::vector<uint32_t> &w) {
+    const size_t ncoeff = 32 * w.size();
     const int nchunks = std::max(1, (int)((ncoeff + kJumpChunk - 1) / kJumpChunk));
-    const size_t base = out.size();
-    out.resize(base + (size_t)nchunks * 64, 0);
-    for (size_t b = 0; b < polys.size(); b++) {
-        const std::vector<uint32_t> &w = *polys[b];
-        for (size_t i = 0; i < w.size(); i++) {
-            uint32_t v = w[i];
-            while (v) {
-                const int bit = __builtin_ctz(v);
-                v &= v - 1;
-                const long k = (long)(32 * i + bit);  // coefficient index
-                const long c = k / kJumpChunk, r = k % kJumpChunk;
-                const long q = r / 64, rho = r % 64;
-                if (c < nchunks)
-                    out[base + (size_t)(c * 64 + rho)] |= (uint64_t)1 << (16 * b + q);
-            }
+    std::vector<uint16_t> m((size_t)nchunks * 64, 0);
+    for (int c = 0; c < nchunks; c++) {
+        // Row q of the chunk = coefficients c*1024 + 64q .. +63; the mask of
+        // residue rho is column rho of this 16 x 64 bit matrix.
+        uint64_t rows[16];
+        for (int q = 0; q < 16; q++) {
+            const size_t i = (size_t)c * 32 + 2 * (size_t)q;
+            const uint64_t lo = i < w.size() ? w[i] : 0u, hi = i + 1 < w.size() ? w[i + 1] : 0u;
+            rows[q] = lo | hi << 32;
+        }
+        uint16_t *mc = m.data() + (size_t)c * 64;
+        for (int rho = 0; rho < 64; rho++) {
+            uint32_t v = 0;
+            for (int q = 0; q < 16; q++)
+                v |= (uint32_t)((rows[q] >> rho) & 1u) << q;
+            mc[rho] = (uint16_t)v;
         }
     }
-    return nchunks;
+    return m;
+}
+
+std::vector<uint32_t> words_from_mask16(const std::vector<uint16_t> &m, int nwords) {
+    std::vector<uint32_t> w((size_t)nwords, 0);
+    const size_t nchunks = m.size() / 64;
+    for (size_t c = 0; c < nchunks; c++) {
+        uint64_t rows[16] = {};
+        const uint16_t *mc = m.data() + c * 64;
+        for (int rho = 0; rho < 64; rho++)
+            for (int q = 0; q < 16; q++)
+                rows[q] |= (uint64_t)((mc[rho] >> q) & 1u) << rho;
+        for (int q = 0; q < 16; q++) {
+            const size_t i = c * 32 + 2 * (size_t)q;
+            if (i < w.size())
+                w[i] = (uint32_t)rows[q];
+            if (i + 1 < w.size())
+                w[i + 1] = (uint32_t)(rows[q] >> 32);
+        }
+    }
+    return w;
+}
+
+int append_jump_masks16(const std::vector<const std::vector<uint16_t> *> &ms, std::vector<uint64_t> &out) {
+    size_t n = 64;
+    for (const auto *m : ms)
+        n = std::max(n, m->size());
+    const size_t base = out.size();
+    out.resize(base + n, 0);
+    uint64_t *o = out.data() + base;
+    for (size_t b = 0; b < ms.size(); b++) {
+        const std::vector<uint16_t> &m = *ms[b];
+        for (size_t i = 0; i < m.size(); i++)
+            o[i] |= (uint64_t)m[i] << (16 * b);
+    }
+    return (int)(n / 64);
+}
+
+int append_jump_masks(const std::vector<const std::vector<uint32_t> *> &polys, std::vector<uint64_t> &out) {
+    std::vector<std::vector<uint16_t>> ms;
+    for (const auto *w : polys)
+        ms.push_back(jump_mask16(*w));
+    std::vector<const std::vector<uint16_t> *> ps;
+    for (const auto &m : ms)
+        ps.push_back(&m);
+    return append_jump_masks16(ps, out);
 }
 
 uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, int slots) {
@@ -401,19 +525,149 @@ uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, 
     return (s + 255) & ~255ull;
 }
 
+namespace {
+gf2::Poly poly_of_words32(const std::vector<uint32_t> &w) {
+    gf2::Poly p((w.size() + 1) / 2, 0);
+    for (size_t i = 0; i < w.size(); i++)
+        p[i / 2] |= (uint64_t)w[i] << (32 * (i % 2));
+    return p;
+}
+}  // namespace
+
+JumpCache::JumpCache(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax) : cell_(cell) {
+    if (cell == 0 || kmax < 1)
+        throw Error("jump cache needs a cell size and kmax >= 1");
+    B_ = (uint64_t)kmax + 1;
+    constexpr int kT2 = 64;
+    std::vector<int> keys(slots);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    std::vector<Entry *> list;
+    for (int s : keys)
+        list.push_back(&entries_[s]);
+    // One task per (slot, block of table entries): the table is a chain of
+    // products, so each block starts from its own x^(k0 cell).
+    constexpr uint64_t kBlock = 64;
+    const uint64_t nblocks = (B_ + kBlock - 1) / kBlock;
+    parallel_for((int)list.size(), [&](int i) {
+        Entry &e = *list[(size_t)i];
+        e.mod = modulus_for(c, keys[(size_t)i]);
+        e.words32 = (int)((e.mod->deg() + 31) / 32);
+        e.pow2.push_back(e.mod->xpow(cell));
+        for (int k = 1; k < 48; k++)
+            e.pow2.push_back(e.mod->sqrmod(e.pow2.back()));
+        e.t1.resize(B_);
+        const gf2::Poly big = e.mod->xpow(std::vector<uint64_t>{B_ * cell});
+        e.t2p.resize(kT2);
+        e.t2p[0] = gf2::Poly{1};
+        e.t2p[1] = big;
+        for (int j = 2; j < kT2; j++)
+            e.t2p[(size_t)j] = e.mod->mulmod(e.t2p[(size_t)j - 1], big);
+    });
+    parallel_for((int)(list.size() * nblocks), [&](int t) {
+        Entry &e = *list[(size_t)t / nblocks];
+        const uint64_t k0 = (uint64_t)t % nblocks * kBlock, k1 = std::min(B_, k0 + kBlock);
+        gf2::Poly cur = k0 ? e.mod->xpow(k0 * cell) : gf2::Poly{1};
+        for (uint64_t k = k0; k < k1; k++) {
+            if (k > 0) {
+                cur = k == k0 ? cur : e.mod->mulmod(cur, e.pow2[0]);
+                e.t1[k] = jump_mask16(gf2::to_words32(cur, e.words32));
+            }
+        }
+    });
+}
+
+namespace {
+std::mutex g_cache_mu;
+std::map<std::tuple<int, uint64_t, int, std::vector<int>>, std::shared_ptr<JumpCache>> g_caches;
+}  // namespace
+
+std::shared_ptr<JumpCache> JumpCache::shared(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax) {
+    std::vector<int> keys(slots);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    auto key = std::make_tuple(c.device, cell, kmax, keys);
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_caches.find(key);
+    if (it != g_caches.end())
+        return it->second;
+    auto p = std::make_shared<JumpCache>(c, keys, cell, kmax);
+    g_caches[key] = p;
+    return p;
+}
+
+bool JumpCache::get_masks(int slot, uint64_t pos, std::vector<uint16_t> &m16, bool cheap) const {
+    if (pos == 0 || pos % cell_)
+        return false;
+    auto it = entries_.find(slot);
+    if (it == entries_.end())
+        return false;
+    const Entry &e = it->second;
+    const uint64_t k = pos / cell_;
+    if (k < B_) {
+        m16 = e.t1[(size_t)k];
+        return true;
+    }
+    if (cheap)
+        return false;
+    const uint64_t j = k / B_, r = k % B_;
+    gf2::Poly acc;
+    if (j < e.t2p.size()) {
+        acc = r ? e.mod->mulmod(e.t2p[(size_t)j], poly_of_words32(words_from_mask16(e.t1[(size_t)r], e.words32)))
+                : e.t2p[(size_t)j];
+    } else {
+        if (k >> e.pow2.size())
+            return false;
+        for (size_t i = 0; i < e.pow2.size(); i++)
+            if (k >> i & 1)
+                acc = acc.empty() ? e.pow2[i] : e.mod->mulmod(acc, e.pow2[i]);
+    }
+    m16 = jump_mask16(gf2::to_words32(acc, e.words32));
+    return true;
+}
+
+bool JumpCache::get(int slot, uint64_t pos, std::vector<uint32_t> &words, bool cheap) const {
+    std::vector<uint16_t> m16;
+    if (!get_masks(slot, pos, m16, cheap))
+        return false;
+    auto it = entries_.find(slot);
+    words = words_from_mask16(m16, it->second.words32);
+    return true;
+}
+
+namespace {
+// Grow-only device / pinned buffers (a plan's buffers are reused by later
+// builds; freeing synchronises, so it only happens when one must grow).
+void grow_device(char *&p, size_t &cap, size_t need, const char *what) {
+    if (need <= cap)
+        return;
+    if (p)
+        hip_check(hipFree(p), "hipFree");
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max(need, (size_t)1 << 20);
+    hip_check(hipMalloc((void **)&p, want), what);
+    cap = want;
+}
+void grow_pinned(char *&p, size_t &cap, size_t need) {
+    if (need <= cap)
+        return;
+    if (p)
+        hip_check(hipHostFree(p), "hipHostFree");
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max(need, (size_t)1 << 20);
+    hip_check(hipHostMalloc((void **)&p, want, hipHostMallocDefault), "hipHostMalloc staging");
+    cap = want;
+}
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+}  // namespace
+
 void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<mtgp_request_t> &reqs,
-                 uint64_t segment) {
+                 const PlanOptions &opt) {
     const auto t0 = std::chrono::steady_clock::now();
     ctx = &c;
-    // Per-generator request lists and the absolute stream offsets.
-    struct Piece {
-        int req;
-        uint64_t off, len, out_off;
-    };
     const int G = (int)gens.size();
-    std::vector<std::vector<Piece>> pieces((size_t)G);
-    std::vector<uint64_t> total((size_t)G, 0);
-    std::vector<char> used((size_t)G, 0);
     uint64_t all = 0;
     for (const auto &r : reqs) {
         if (r.gen < 0 || r.gen >= G)
@@ -433,53 +687,136 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     int slots = 0;
     for (const auto &g : gens) {
         const int b = g.bits == 64, cls = kernel_class(g.bits, g.n);
-        slots = std::max(slots, c.cus * std::min(c.default_bpc, c.blocks_per_cu[b][cls][c.layout[b][cls]]));
+        slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][c.layout[b][cls]]));
     }
-    const uint64_t S = segment ? segment : auto_segment(c, all, nonempty, slots);
+    const uint64_t S = opt.cache ? opt.cache->cell() : opt.segment ? opt.segment : auto_segment(c, all, nonempty, slots);
+    // Requests as runs of stream positions per generator.
+    struct Run {
+        int req;
+        uint64_t pos, count;
+    };
+    std::vector<std::vector<Run>> runs((size_t)G);
+    std::vector<uint64_t> total((size_t)G, 0);
+    std::vector<char> used((size_t)G, 0);
     for (int ri = 0; ri < (int)reqs.size(); ri++) {
         const auto &r = reqs[(size_t)ri];
         uint64_t &pos = total[(size_t)r.gen];
         used[(size_t)r.gen] = 1;
         pos += r.skip;
-        for (uint64_t done = 0; done < r.count;) {
-            const uint64_t len = std::min<uint64_t>(S, r.count - done);
-            pieces[(size_t)r.gen].push_back(Piece{ri, pos + done, len, done});
-            done += len;
-        }
+        if (r.count)
+            runs[(size_t)r.gen].push_back(Run{ri, pos, r.count});
         pos += r.count;
     }
-    // A pure skip at the end still has to move the state.
+    // Cells.
+    struct Piece {
+        int req;  // -1: stepped over
+        uint64_t out_off, count;
+    };
+    struct Cell {
+        uint64_t start, end;
+        std::vector<Piece> pieces;
+    };
+    std::vector<std::vector<Cell>> cells((size_t)G);
     for (int g = 0; g < G; g++) {
         if (!used[(size_t)g])
             continue;
-        auto &pv = pieces[(size_t)g];
-        if (pv.empty() || pv.back().off + pv.back().len != total[(size_t)g])
-            pv.push_back(Piece{-1, total[(size_t)g], 0, 0});
+        auto &cv = cells[(size_t)g];
+        // Continue the current cell at `pos` (stepping over a short gap) or
+        // open a new one: at its grid point when the gap to it is short, else
+        // exactly at pos.
+        auto at = [&](uint64_t pos) {
+            if (!cv.empty()) {
+                Cell &cur = cv.back();
+                if (pos / S == cur.start / S && pos >= cur.end && pos - cur.end <= opt.discard_max) {
+                    if (pos > cur.end)
+                        cur.pieces.push_back(Piece{-1, 0, pos - cur.end});
+                    cur.end = pos;
+                    return;
+                }
+            }
+            const uint64_t g0 = pos / S * S;
+            uint64_t start = (pos - g0 <= opt.discard_max) ? g0 : pos;
+            if (!cv.empty() && start < cv.back().end)
+                start = pos;
+            Cell cell;
+            cell.start = start;
+            cell.end = pos;
+            if (pos > start)
+                cell.pieces.push_back(Piece{-1, 0, pos - start});
+            cv.push_back(std::move(cell));
+        };
+        for (const Run &r : runs[(size_t)g]) {
+            uint64_t done = 0;
+            while (done < r.count) {
+                const uint64_t pos = r.pos + done;
+                const uint64_t take = std::min(r.count - done, (pos / S + 1) * S - pos);
+                at(pos);
+                cv.back().pieces.push_back(Piece{r.req, done, take});
+                cv.back().end = pos + take;
+                done += take;
+            }
+        }
+        if (cv.empty() || cv.back().end != total[(size_t)g])
+            at(total[(size_t)g]);
     }
-    // Jump polynomials x^off mod P for every piece with off > 0.
+    // Jump polynomials x^start mod P for cells with start > 0, as
+    // coefficient words (fused jumps, mode 0) or in per-residue mask form
+    // (batched jumps, mode 1): from the cache when it has them (table
+    // entries inline, one-product entries on host threads), else
+    // incrementally along each generator on host threads.
+    if (opt.jump_mode >= 0)
+        jump_mode = opt.jump_mode ? 1 : 0;
+    else if (const char *e = getenv("MTGP_JUMP_MODE"))
+        jump_mode = atoi(e) ? 1 : 0;
+    const bool masks = jump_mode == 1;
     std::vector<std::vector<std::vector<uint32_t>>> polys((size_t)G);
-    parallel_for(G, [&](int g) {
-        const auto &pv = pieces[(size_t)g];
+    std::vector<std::vector<std::vector<uint16_t>>> m16((size_t)G);
+    auto cached = [&](int g, size_t k, bool cheap) {
+        const int slot = gens[(size_t)g].slot;
+        const uint64_t pos = cells[(size_t)g][k].start;
+        return masks ? opt.cache->get_masks(slot, pos, m16[(size_t)g][k], cheap)
+                     : opt.cache->get(slot, pos, polys[(size_t)g][k], cheap);
+    };
+    std::vector<int> compute;                      // generators with cells off the cache
+    std::vector<std::pair<int, size_t>> composed;  // cells the cache makes with one product
+    for (int g = 0; g < G; g++) {
+        const auto &cv = cells[(size_t)g];
+        polys[(size_t)g].resize(cv.size());
+        m16[(size_t)g].resize(cv.size());
+        bool need = false;
+        for (size_t k = 0; k < cv.size(); k++) {
+            if (cv[k].start == 0)
+                continue;
+            if (opt.cache && cached(g, k, true))
+                continue;
+            if (opt.cache && cv[k].start % S == 0)
+                composed.emplace_back(g, k);
+            else
+                need = true;
+        }
+        if (need)
+            compute.push_back(g);
+    }
+    parallel_for((int)composed.size(), [&](int i) {
+        (void)cached(composed[(size_t)i].first, composed[(size_t)i].second, false);
+    });
+    parallel_for((int)compute.size(), [&](int i) {
+        const int g = compute[(size_t)i];
+        const auto &cv = cells[(size_t)g];
         auto &out = polys[(size_t)g];
-        out.resize(pv.size());
-        bool any = false;
-        for (const auto &p : pv)
-            any |= p.off > 0;
-        if (!any)
-            return;
         const GenRef &gr = gens[(size_t)g];
         auto mod = modulus_for(c, gr.slot);
         const int pw = (gr.mexp + 31) / 32;
         std::map<uint64_t, gf2::Poly> step_cache;
         gf2::Poly cur;
         uint64_t cur_off = 0;
-        for (size_t k = 0; k < pv.size(); k++) {
-            const uint64_t off = pv[k].off;
-            if (off == 0)
+        for (size_t k = 0; k < cv.size(); k++) {
+            const uint64_t off = cv[k].start;
+            if (off == 0 || !out[k].empty() || !m16[(size_t)g][k].empty())
                 continue;
             if (cur_off == 0) {
                 cur = mod->xpow(off);
-            } else if (off != cur_off) {
+            } else {
                 const uint64_t delta = off - cur_off;
                 auto it = step_cache.find(delta);
                 if (it == step_cache.end())
@@ -488,45 +825,35 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             }
             cur_off = off;
             out[k] = gf2::to_words32(cur, pw);
+            if (masks)
+                m16[(size_t)g][k] = jump_mask16(out[k]);
         }
     });
-    // Items, grouped by (word size, kernel class), longest first.  With
-    // jump_mode 1 the start states of pieces at off > 0 come from batched
-    // jump items (kJumpBatch per item, all from the generator's state)
-    // that write them to a state arena.
-    if (const char *e = getenv("MTGP_JUMP_MODE"))
-        jump_mode = atoi(e) ? 1 : 0;
-    std::vector<Item> list;
-    std::vector<int> group;
-    std::vector<uint32_t> poly_host;
-    std::vector<uint64_t> mask_host;
-    std::vector<int> jgroup;
-    std::vector<uint64_t> arena_slot(0);  // per list item: arena offset + 1 (0 = none)
-    size_t arena_bytes = 0;
-    jitems.clear();
-    // Jump groups first (to size the parts), then items.
+    // Jump groups (jump_mode 1): up to kJumpBatch cells of one generator.
     struct Group {
         int g;
-        std::vector<size_t> pieces;
-        size_t mask_off;  // byte offset into the mask arena
+        std::vector<size_t> cells;
+        size_t mask_off;  // byte offset into the mask array
         int nchunks;
     };
     std::vector<Group> groups;
-    if (jump_mode == 1) {
+    std::vector<uint64_t> mask_host;
+    std::vector<uint32_t> poly_host;
+    if (masks) {
         for (int g = 0; g < G; g++) {
             std::vector<size_t> need;
-            for (size_t k = 0; k < pieces[(size_t)g].size(); k++)
-                if (pieces[(size_t)g][k].off > 0)
+            for (size_t k = 0; k < cells[(size_t)g].size(); k++)
+                if (cells[(size_t)g][k].start > 0)
                     need.push_back(k);
             for (size_t i0 = 0; i0 < need.size(); i0 += kJumpBatch) {
                 Group gr;
                 gr.g = g;
-                gr.pieces.assign(need.begin() + (long)i0, need.begin() + (long)std::min(need.size(), i0 + kJumpBatch));
-                std::vector<const std::vector<uint32_t> *> ps;
-                for (size_t k : gr.pieces)
-                    ps.push_back(&polys[(size_t)g][k]);
+                gr.cells.assign(need.begin() + (long)i0, need.begin() + (long)std::min(need.size(), i0 + kJumpBatch));
+                std::vector<const std::vector<uint16_t> *> ps;
+                for (size_t k : gr.cells)
+                    ps.push_back(&m16[(size_t)g][k]);
                 gr.mask_off = mask_host.size() * 8;
-                gr.nchunks = append_jump_masks(ps, mask_host);
+                gr.nchunks = append_jump_masks16(ps, mask_host);
                 groups.push_back(gr);
             }
         }
@@ -538,15 +865,19 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         const int target = c.cus * 16;
         parts = std::max(1, std::min<int>((target + (int)groups.size() - 1) / (int)groups.size(), 8));
     }
-    std::map<std::pair<int, size_t>, std::pair<uint64_t, int>> slot_of;  // (g, piece) -> (arena off, part stride)
+    std::vector<int> jgroup;
+    jitems.clear();
+    size_t arena_bytes = 0;
+    // (g, cell) -> (arena offset, parts, part stride)
+    std::map<std::pair<int, size_t>, std::tuple<uint64_t, int, uint64_t>> slot_of;
     for (const Group &gr : groups) {
         const GenRef &ref = gens[(size_t)gr.g];
         const size_t sbytes = (size_t)ref.n * (size_t)(ref.bits / 8);
-        const size_t nb = gr.pieces.size();
+        const size_t nb = gr.cells.size();
         const int P = std::min(parts, gr.nchunks);
         const size_t base = arena_bytes;
         for (size_t b = 0; b < nb; b++)
-            slot_of[{gr.g, gr.pieces[b]}] = {base + b * sbytes, P};
+            slot_of[{gr.g, gr.cells[b]}] = std::make_tuple((uint64_t)(base + b * sbytes), P, (uint64_t)(nb * sbytes));
         for (int q = 0; q < P; q++) {
             JumpItem ji{};
             ji.state_in = (uint64_t)(uintptr_t)ref.state_in;
@@ -560,34 +891,59 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             jitems.push_back(ji);
             jgroup.push_back((ref.bits == 64 ? 3 : 0) + kernel_class(ref.bits, ref.n));
         }
-        arena_bytes += (((size_t)P * nb * sbytes + 255) & ~(size_t)255);
+        arena_bytes += align256((size_t)P * nb * sbytes);
     }
+    // Items, one per cell.
+    std::vector<Item> list;
+    std::vector<int> group;
+    std::vector<uint64_t> arena_slot;  // per item: arena offset + 1 (0 = none)
+    std::vector<uint64_t> cost;
+    segs.clear();
+    commits.clear();
+    numbers = 0;
+    jumps = 0;
     for (int g = 0; g < G; g++) {
         const GenRef &gr = gens[(size_t)g];
-        const size_t sbytes = (size_t)gr.n * (size_t)(gr.bits / 8);
-        for (size_t k = 0; k < pieces[(size_t)g].size(); k++) {
-            const Piece &p = pieces[(size_t)g][k];
+        const auto &cv = cells[(size_t)g];
+        for (size_t k = 0; k < cv.size(); k++) {
+            const Cell &cell = cv[k];
             Item it{};
-            it.count = p.len;
-            it.kind = p.req >= 0 ? reqs[(size_t)p.req].kind : 0;
-            it.out = p.req >= 0 ? (uint64_t)(uintptr_t)reqs[(size_t)p.req].out + p.out_off * (uint64_t)elem_bytes(gr.bits)
-                                : 0;
             it.state_in = (uint64_t)(uintptr_t)gr.state_in;
-            it.state_out = (p.off + p.len == total[(size_t)g]) ? (uint64_t)(uintptr_t)gr.state_out : 0;
+            it.state_out = (k + 1 == cv.size()) ? (uint64_t)(uintptr_t)gr.state_out : 0;
             it.param = gr.slot;
             it.poly = UINT64_MAX;
+            uint64_t n = 0;
+            for (const Piece &pc : cell.pieces)
+                n += pc.count;
+            auto seg_of = [&](const Piece &pc) {
+                Segment sg{};
+                sg.count = pc.count;
+                if (pc.req >= 0) {
+                    const auto &r = reqs[(size_t)pc.req];
+                    sg.out = (uint64_t)(uintptr_t)r.out + pc.out_off * (uint64_t)elem_bytes(gr.bits);
+                    sg.kind = r.kind;
+                }
+                return sg;
+            };
+            if (cell.pieces.size() == 1 && cell.pieces[0].req >= 0) {
+                const Segment sg = seg_of(cell.pieces[0]);
+                it.out = sg.out;
+                it.count = sg.count;
+                it.kind = sg.kind;
+            } else if (!cell.pieces.empty()) {
+                it.segs = segs.size();  // index, fixed up below
+                it.nsegs = (int32_t)cell.pieces.size();
+                for (const Piece &pc : cell.pieces)
+                    segs.push_back(seg_of(pc));
+            }
             uint64_t slot = 0;
-            if (p.off > 0) {
+            if (cell.start > 0) {
+                jumps++;
                 if (jump_mode == 1) {
                     const auto &sp = slot_of.at({g, k});
-                    slot = sp.first + 1;
-                    it.parts = sp.second;
-                    // partial windows of part q sit q * nb * sbytes apart
-                    size_t nb = 0;
-                    for (const Group &grp : groups)
-                        if (grp.g == g && std::find(grp.pieces.begin(), grp.pieces.end(), k) != grp.pieces.end())
-                            nb = grp.pieces.size();
-                    it.part_bytes = nb * sbytes;
+                    slot = std::get<0>(sp) + 1;
+                    it.parts = std::get<1>(sp);
+                    it.part_bytes = std::get<2>(sp);
                 } else {
                     const auto &pw = polys[(size_t)g][k];
                     it.poly = poly_host.size();  // word offset, fixed up below
@@ -595,106 +951,119 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
                     poly_host.insert(poly_host.end(), pw.begin(), pw.end());
                 }
             }
+            numbers += n;
             list.push_back(it);
             arena_slot.push_back(slot);
+            cost.push_back(n + (it.poly != UINT64_MAX ? 200000u : 0u));
             group.push_back((gr.bits == 64 ? 3 : 0) + kernel_class(gr.bits, gr.n));
             if (it.state_out)
                 commits.push_back(g);
         }
     }
+    // Longest first within each (word size, class) group.
     std::vector<int> order(list.size());
     for (size_t i = 0; i < order.size(); i++)
         order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         if (group[(size_t)a] != group[(size_t)b])
             return group[(size_t)a] < group[(size_t)b];
-        // cost ~ numbers + a jump's worth when there is one
-        const uint64_t ca = list[(size_t)a].count + (list[(size_t)a].poly != UINT64_MAX ? 200000u : 0u);
-        const uint64_t cb = list[(size_t)b].count + (list[(size_t)b].poly != UINT64_MAX ? 200000u : 0u);
-        return ca > cb;
+        return cost[(size_t)a] > cost[(size_t)b];
     });
-    // Device copies.
-    free_device();
-    if (!poly_host.empty()) {
-        hip_check(hipMalloc((void **)&d_polys, poly_host.size() * 4), "hipMalloc polys");
-        hip_check(hipMemcpy(d_polys, poly_host.data(), poly_host.size() * 4, hipMemcpyHostToDevice), "hipMemcpy polys");
-    }
-    if (!mask_host.empty()) {
-        hip_check(hipMalloc((void **)&d_masks, mask_host.size() * 8), "hipMalloc masks");
-        hip_check(hipMemcpy(d_masks, mask_host.data(), mask_host.size() * 8, hipMemcpyHostToDevice), "hipMemcpy masks");
-        hip_check(hipMalloc((void **)&d_jstates, arena_bytes), "hipMalloc state arena");
-    }
-    for (auto &ji : jitems) {
-        ji.masks = (uint64_t)(uintptr_t)d_masks + ji.masks;
-        ji.states_out = (uint64_t)(uintptr_t)d_jstates + ji.states_out;
-    }
-    items.clear();
-    for (int i : order) {
-        Item it = list[(size_t)i];
-        it.poly = it.poly == UINT64_MAX ? 0 : (uint64_t)(uintptr_t)(d_polys + it.poly);
-        if (arena_slot[(size_t)i])
-            it.state_in = (uint64_t)(uintptr_t)d_jstates + arena_slot[(size_t)i] - 1;
-        items.push_back(it);
-    }
-    // Jump launches, one per (word size, class), before the generation.
-    jlaunches.clear();
     {
         std::vector<int> jorder(jitems.size());
         for (size_t i = 0; i < jorder.size(); i++)
             jorder[i] = (int)i;
         std::stable_sort(jorder.begin(), jorder.end(), [&](int a, int b) { return jgroup[(size_t)a] < jgroup[(size_t)b]; });
         std::vector<JumpItem> sorted;
-        for (int i : jorder)
+        std::vector<int> sg;
+        for (int i : jorder) {
             sorted.push_back(jitems[(size_t)i]);
+            sg.push_back(jgroup[(size_t)i]);
+        }
         jitems.swap(sorted);
-        for (size_t i = 0; i < jorder.size();) {
-            const int gi = jgroup[(size_t)jorder[i]];
+        jgroup.swap(sg);
+    }
+    // Workspace layout and device addresses.
+    const size_t o_items = 0;
+    const size_t o_segs = align256(o_items + list.size() * sizeof(Item));
+    const size_t o_polys = align256(o_segs + segs.size() * sizeof(Segment));
+    const size_t o_masks = align256(o_polys + poly_host.size() * 4);
+    const size_t o_jitems = align256(o_masks + mask_host.size() * 8);
+    const size_t work_bytes = align256(o_jitems + jitems.size() * sizeof(JumpItem));
+    grow_device(d_work, work_cap, work_bytes, "hipMalloc plan workspace");
+    grow_pinned(h_stage, stage_cap, work_bytes);
+    if (arena_bytes)
+        grow_device(d_jstates, jstates_cap, arena_bytes, "hipMalloc state arena");
+    if (!d_queue) {
+        hip_check(hipMalloc((void **)&d_queue, 64 * sizeof(uint32_t)), "hipMalloc queue");
+        hip_check(hipMemset(d_queue, 0, 64 * sizeof(uint32_t)), "hipMemset queue");
+    }
+    const uint64_t dw = (uint64_t)(uintptr_t)d_work;
+    d_items = reinterpret_cast<Item *>(d_work + o_items);
+    d_jitems = reinterpret_cast<JumpItem *>(d_work + o_jitems);
+    items.clear();
+    for (int i : order) {
+        Item it = list[(size_t)i];
+        it.poly = it.poly == UINT64_MAX ? 0 : dw + o_polys + 4 * it.poly;
+        if (it.nsegs > 0)
+            it.segs = dw + o_segs + sizeof(Segment) * it.segs;
+        if (arena_slot[(size_t)i])
+            it.state_in = (uint64_t)(uintptr_t)d_jstates + arena_slot[(size_t)i] - 1;
+        items.push_back(it);
+    }
+    for (auto &ji : jitems) {
+        ji.masks = dw + o_masks + ji.masks;
+        ji.states_out = (uint64_t)(uintptr_t)d_jstates + ji.states_out;
+    }
+    if (!items.empty())
+        memcpy(h_stage + o_items, items.data(), items.size() * sizeof(Item));
+    if (!segs.empty())
+        memcpy(h_stage + o_segs, segs.data(), segs.size() * sizeof(Segment));
+    if (!poly_host.empty())
+        memcpy(h_stage + o_polys, poly_host.data(), poly_host.size() * 4);
+    if (!mask_host.empty())
+        memcpy(h_stage + o_masks, mask_host.data(), mask_host.size() * 8);
+    if (!jitems.empty())
+        memcpy(h_stage + o_jitems, jitems.data(), jitems.size() * sizeof(JumpItem));
+    hip_check(hipMemcpyAsync(d_work, h_stage, work_bytes, hipMemcpyHostToDevice, opt.stream), "plan upload");
+    if (opt.sync)
+        hip_check(hipStreamSynchronize(opt.stream), "plan upload sync");
+    // Launches: one jump launch and one generation launch per (word size,
+    // class) group.
+    auto group_launches = [&](const std::vector<int> &grp, bool jump) {
+        std::vector<Launch> out;
+        for (size_t i = 0; i < grp.size();) {
+            const int gi = grp[i];
             size_t j = i;
-            while (j < jorder.size() && jgroup[(size_t)jorder[j]] == gi)
+            while (j < grp.size() && grp[j] == gi)
                 j++;
             Launch L;
             L.bits = gi >= 3 ? 64 : 32;
             L.cls = gi % 3;
-            L.layout = 1;
             L.first = (int)i;
             L.count = (int)(j - i);
-            L.grid = std::min(L.count, c.cus * c.jump_blocks_per_cu[L.bits == 64][L.cls]);
-            L.dyn_lds = 0;
-            jlaunches.push_back(L);
+            if (jump) {
+                L.layout = 1;
+                L.grid = std::min(L.count, c.cus * c.jump_blocks_per_cu[L.bits == 64][L.cls]);
+                L.dyn_lds = 0;
+            } else {
+                L.layout = c.layout[L.bits == 64][L.cls];
+                const int bpc = std::min(c.bpc[L.bits == 64][L.cls], c.blocks_per_cu[L.bits == 64][L.cls][L.layout]);
+                L.grid = std::min(L.count, c.cus * bpc);
+                L.dyn_lds = c.lds_reserve(kernel_fn(L.bits, L.cls, L.layout), bpc);
+            }
+            out.push_back(L);
             i = j;
         }
-    }
-    if (!jitems.empty()) {
-        hip_check(hipMalloc((void **)&d_jitems, jitems.size() * sizeof(JumpItem)), "hipMalloc jump items");
-        hip_check(hipMemcpy(d_jitems, jitems.data(), jitems.size() * sizeof(JumpItem), hipMemcpyHostToDevice),
-                  "hipMemcpy jump items");
-    }
-    launches.clear();
-    for (size_t i = 0; i < order.size();) {
-        const int gi = group[(size_t)order[i]];
-        size_t j = i;
-        while (j < order.size() && group[(size_t)order[j]] == gi)
-            j++;
-        Launch L;
-        L.bits = gi >= 3 ? 64 : 32;
-        L.cls = gi % 3;
-        L.layout = c.layout[L.bits == 64][L.cls];
-        L.first = (int)i;
-        L.count = (int)(j - i);
-        const int bpc = std::min(c.default_bpc, c.blocks_per_cu[L.bits == 64][L.cls][L.layout]);
-        L.grid = std::min(L.count, c.cus * bpc);
-        L.dyn_lds = c.lds_reserve(kernel_fn(L.bits, L.cls, L.layout), bpc);
-        launches.push_back(L);
-        i = j;
-    }
-    if (!items.empty()) {
-        hip_check(hipMalloc((void **)&d_items, items.size() * sizeof(Item)), "hipMalloc items");
-        hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice),
-                  "hipMemcpy items");
-        const size_t nq = (launches.size() + jlaunches.size()) * 2 * sizeof(uint32_t);
-        hip_check(hipMalloc((void **)&d_queue, nq), "hipMalloc queue");
-        hip_check(hipMemset(d_queue, 0, nq), "hipMemset queue");
-    }
+        return out;
+    };
+    std::vector<int> sorted_group;
+    for (int i : order)
+        sorted_group.push_back(group[(size_t)i]);
+    launches = group_launches(sorted_group, false);
+    jlaunches = group_launches(jgroup, true);
+    if (2 * (launches.size() + jlaunches.size()) > 64)
+        throw Error("too many launches in one plan");
     segment_used = S;
     plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -721,6 +1090,7 @@ void Plan::launch_part(int part, hipStream_t s) {
 }
 
 void Plan::configure(int layout, int bpc) {
+    drop_graph();
     for (auto &L : launches) {
         if (layout >= 0) {
             if (layout >= kLayouts)
@@ -728,31 +1098,116 @@ void Plan::configure(int layout, int bpc) {
             L.layout = layout;
         }
         int cap = ctx->blocks_per_cu[L.bits == 64][L.cls][L.layout];
-        cap = std::min(cap, bpc > 0 ? bpc : ctx->default_bpc);
+        cap = std::min(cap, bpc > 0 ? bpc : ctx->bpc[L.bits == 64][L.cls]);
         L.grid = std::min(L.count, ctx->cus * cap);
         L.dyn_lds = ctx->lds_reserve(kernel_fn(L.bits, L.cls, L.layout), cap);
     }
 }
 
+void Plan::drop_graph() {
+    if (graph_exec)
+        (void)hipGraphExecDestroy(graph_exec);
+    if (graph)
+        (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
+void Plan::capture(hipStream_t s) {
+    if (!s)
+        throw Error("capture needs a stream other than the null stream");
+    drop_graph();
+    hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        launch(s);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g)
+            (void)hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+void Plan::launch_graph(hipStream_t s) {
+    if (!graph_exec)
+        throw Error("plan has no captured graph");
+    hip_check(hipGraphLaunch(graph_exec, s), "hipGraphLaunch");
+}
+
+float Plan::autotune(hipStream_t s, int reps, int *best_layout, int *best_bpc) {
+    if (launches.empty())
+        return 0.f;
+    reps = std::max(1, reps);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    float best = -1.f;
+    int bl = -1, bb = -1;
+    try {
+        launch_part(0, s);  // start states stay valid for every trial
+        for (int layout = 0; layout < kLayouts; layout++) {
+            int last_cap = -1;
+            for (int bpc : {4, 6, 8, 12, 16, 20, 24, 32}) {
+                int cap = bpc;
+                for (const auto &L : launches)
+                    cap = std::min(cap, ctx->blocks_per_cu[L.bits == 64][L.cls][layout]);
+                if (cap == last_cap)
+                    continue;
+                last_cap = cap;
+                configure(layout, cap);
+                launch_part(1, s);  // warm-up
+                hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                for (int r = 0; r < reps; r++)
+                    launch_part(1, s);
+                hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                ms /= (float)reps;
+                if (best < 0.f || ms < best) {
+                    best = ms;
+                    bl = layout;
+                    bb = cap;
+                }
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    configure(bl, bb);
+    for (const auto &L : launches) {
+        ctx->layout[L.bits == 64][L.cls] = bl;
+        ctx->bpc[L.bits == 64][L.cls] = bb;
+    }
+    if (best_layout)
+        *best_layout = bl;
+    if (best_bpc)
+        *best_bpc = bb;
+    return best;
+}
+
 void Plan::free_device() {
-    if (d_jitems)
-        (void)hipFree(d_jitems);
-    if (d_masks)
-        (void)hipFree(d_masks);
+    drop_graph();
+    if (d_work)
+        (void)hipFree(d_work);
     if (d_jstates)
         (void)hipFree(d_jstates);
-    d_jitems = nullptr;
-    d_masks = nullptr;
-    d_jstates = nullptr;
-    if (d_items)
-        (void)hipFree(d_items);
-    if (d_polys)
-        (void)hipFree(d_polys);
     if (d_queue)
         (void)hipFree(d_queue);
-    d_items = nullptr;
-    d_polys = nullptr;
+    if (h_stage)
+        (void)hipHostFree(h_stage);
+    d_work = d_jstates = h_stage = nullptr;
     d_queue = nullptr;
+    work_cap = jstates_cap = stage_cap = 0;
+    d_items = nullptr;
+    d_jitems = nullptr;
 }
 
 Plan::~Plan() { free_device(); }
@@ -770,32 +1225,6 @@ using namespace mtgp;
         set_error(e.what());            \
         return fail;                    \
     }
-
-struct mtgp_genset {
-    Context *ctx;
-    std::vector<int> bits, mexp, slot, n, cur;
-    std::vector<size_t> off;  // byte offset of slot pair in d_states
-    char *d_states = nullptr;
-    ~mtgp_genset() {
-        if (d_states)
-            (void)hipFree(d_states);
-    }
-    void *state(int g, int which) {
-        return d_states + off[(size_t)g] + (size_t)which * (size_t)n[(size_t)g] * (size_t)(bits[(size_t)g] / 8);
-    }
-    std::vector<GenRef> refs() {
-        std::vector<GenRef> r(bits.size());
-        for (size_t g = 0; g < bits.size(); g++) {
-            r[g].bits = bits[g];
-            r[g].mexp = mexp[g];
-            r[g].n = n[g];
-            r[g].slot = slot[g];
-            r[g].state_in = state((int)g, cur[g]);
-            r[g].state_out = state((int)g, cur[g] ^ 1);
-        }
-        return r;
-    }
-};
 
 struct mtgp_plan {
     mtgp_genset *set;
@@ -1253,6 +1682,29 @@ int mtgp_plan_launch_part(mtgp_plan_t *p, int part, void *stream) {
 int mtgp_plan_configure(mtgp_plan_t *p, int layout, int blocks_per_cu) {
     API_BEGIN
     p->plan.configure(layout, blocks_per_cu);
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_plan_capture(mtgp_plan_t *p, void *stream) {
+    API_BEGIN
+    p->plan.capture((hipStream_t)stream);
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_plan_launch_graph(mtgp_plan_t *p, void *stream) {
+    API_BEGIN
+    p->plan.launch_graph((hipStream_t)stream);
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_plan_autotune(mtgp_plan_t *p, void *stream, int reps, int *layout, int *blocks_per_cu, float *best_ms) {
+    API_BEGIN
+    const float ms = p->plan.autotune((hipStream_t)stream, reps, layout, blocks_per_cu);
+    if (best_ms)
+        *best_ms = ms;
     return 0;
     API_END(-1)
 }

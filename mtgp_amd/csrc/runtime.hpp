@@ -7,6 +7,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "mtgp32-fast.h"
@@ -39,6 +40,9 @@ struct Context {
     // Resident workgroups per CU used by default (measured best for the
     // generation kernels: 16 of the 32 the occupancy limit allows).
     int default_bpc = 16;
+    // Resident workgroups per CU for generation launches per [bits==64][class]
+    // (default_bpc until the autotuner measures better).
+    int bpc[2][3] = {{16, 16, 16}, {16, 16, 16}};
     // Automatic segmentation cuts the work of a plan into about this many
     // segments per resident workgroup slot: more rounds balance the tail
     // better, each extra segment costs one jump-ahead (1.5 measured best for
@@ -71,8 +75,14 @@ DevParam dev_param_of(const mtgp64_params_fast_t &p);
 std::shared_ptr<gf2::Modulus> modulus_for(const DevParam &d, const unsigned char *sha1);
 std::shared_ptr<gf2::Modulus> modulus_for(Context &c, int slot);
 
-// Per-residue coefficient masks of up to kJumpBatch jump polynomials
-// (layout in common.hpp), appended to `out`; returns the chunk count.
+// Per-residue 16-bit form of one jump polynomial given as 32-bit
+// coefficient words: m[c * 64 + rho] bit q = coefficient of
+// x^(c * kJumpChunk + 64 q + rho).  And back.
+std::vector<uint16_t> jump_mask16(const std::vector<uint32_t> &words);
+std::vector<uint32_t> words_from_mask16(const std::vector<uint16_t> &m, int nwords);
+// Per-residue coefficient masks of up to kJumpBatch jumps (layout in
+// common.hpp), appended to `out`; returns the chunk count.
+int append_jump_masks16(const std::vector<const std::vector<uint16_t> *> &ms, std::vector<uint64_t> &out);
 int append_jump_masks(const std::vector<const std::vector<uint32_t> *> &polys, std::vector<uint64_t> &out);
 
 // A generator as the planner sees it.
@@ -84,38 +94,111 @@ struct GenRef {
 
 uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, int slots);
 
+// Jump polynomials x^pos mod P kept per parameter slot for positions on a
+// grid of `cell` numbers (pos = k * cell): table T1[k] = x^(k cell) for
+// k < B = kmax + 1, and T2[j] = x^(j B cell) for j < 64, so any k < 64 B
+// costs at most one multiplication; x^(2^i cell) composes larger k.
+// Read-only after construction, so planners on several threads may share
+// it.
+class JumpCache {
+  public:
+    JumpCache(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax);
+    // One cache per (device, cell, kmax, slots) for the whole process.
+    static std::shared_ptr<JumpCache> shared(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax);
+    uint64_t cell() const { return cell_; }
+    // x^pos mod P_slot in the per-residue mask form (jump_mask16), when pos
+    // is a positive multiple of the cell size and the slot is cached; false
+    // otherwise.  `cheap` limits the lookup to table entries (no product).
+    bool get_masks(int slot, uint64_t pos, std::vector<uint16_t> &m16, bool cheap = false) const;
+    // The same as coefficient words.
+    bool get(int slot, uint64_t pos, std::vector<uint32_t> &words, bool cheap = false) const;
+
+  private:
+    struct Entry {
+        std::shared_ptr<gf2::Modulus> mod;
+        int words32 = 0;
+        std::vector<std::vector<uint16_t>> t1;  // [k] = x^(k cell) in mask form, k < B
+        std::vector<gf2::Poly> t2p;              // [j] = x^(j B cell)
+        std::vector<gf2::Poly> pow2;             // [i] = x^(2^i cell)
+    };
+    uint64_t cell_;
+    uint64_t B_ = 0;
+    std::map<int, Entry> entries_;
+};
+
+struct PlanOptions {
+    uint64_t segment = 0;          // numbers per grid cell (0: automatic)
+    uint64_t discard_max = 65536;  // gaps up to this size are stepped over, not jumped
+    const JumpCache *cache = nullptr;
+    hipStream_t stream = nullptr;  // upload stream
+    bool sync = true;              // wait for the upload before returning
+    int jump_mode = -1;            // 0 fused, 1 batched, -1 the plan's default
+};
+
 // A list of device Items plus the launches that run them.
+//
+// Each generator's requests occupy consecutive positions of its stream
+// (skips leave gaps).  The positions are cut on a grid of S numbers into
+// cells; a cell is one Item: a jump to the cell's start (or the generator's
+// state for the first cell) and then its segments in order -- pieces of
+// requests, or numbers stepped over -- so consecutive requests need no jump
+// between them.
 struct Plan {
     struct Launch {
         int bits, cls, layout, first, count, grid;
         size_t dyn_lds;
     };
     Context *ctx = nullptr;
-    // Start states of segments: 0 = jump inside the generation kernel
+    // Start states of cells: 0 = jump inside the generation kernel
     // (Item.poly), 1 = a batched jump launch first (JumpItem) writing them
     // to a state arena, so the generation launch is jump-free.
     int jump_mode = 1;
     std::vector<Item> items;
+    std::vector<Segment> segs;
     std::vector<Launch> launches;
     std::vector<JumpItem> jitems;
     std::vector<Launch> jlaunches;  // layout unused
-    JumpItem *d_jitems = nullptr;
-    uint64_t *d_masks = nullptr;
+    std::vector<int> commits;       // generators whose state moves
+    // Device copies: one grow-only workspace [items | segs | polys | masks |
+    // jump items], the jumped-state arena and the work-queue counters.
+    char *d_work = nullptr;
+    size_t work_cap = 0;
+    char *h_stage = nullptr;  // pinned staging copy of the workspace
+    size_t stage_cap = 0;
     char *d_jstates = nullptr;
-    std::vector<int> commits;  // generators whose state moves
-    Item *d_items = nullptr;
-    uint32_t *d_polys = nullptr;
+    size_t jstates_cap = 0;
     uint32_t *d_queue = nullptr;
+    Item *d_items = nullptr;
+    JumpItem *d_jitems = nullptr;
     uint64_t segment_used = 0;
+    uint64_t numbers = 0;  // numbers generated or stepped over
+    int jumps = 0;         // cells that start with a jump
     double plan_ms = 0;
 
     void build(Context &c, const std::vector<GenRef> &gens, const std::vector<mtgp_request_t> &reqs,
-               uint64_t segment);
+               const PlanOptions &opt);
+    void build(Context &c, const std::vector<GenRef> &gens, const std::vector<mtgp_request_t> &reqs,
+               uint64_t segment) {
+        PlanOptions o;
+        o.segment = segment;
+        build(c, gens, reqs, o);
+    }
     void launch(hipStream_t s);
     void launch_part(int part, hipStream_t s);  // 0 jumps, 1 generation
     // Overrides the step layout and/or the resident workgroups per CU of
     // every launch (-1 keeps the current value).
     void configure(int layout, int blocks_per_cu);
+    // Records the plan's launches in a hipGraph (on s, not the null stream)
+    // and instantiates it; launch_graph replays it.  configure drops it.
+    void capture(hipStream_t s);
+    void launch_graph(hipStream_t s);
+    void drop_graph();
+    // Times the generation launches for every step layout and resident
+    // workgroups per CU (after one jump launch), configures the fastest and
+    // makes it the default of its classes.  Returns the best time in ms.
+    float autotune(hipStream_t s, int reps, int *layout, int *bpc);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     void free_device();
     Plan() = default;
     Plan(const Plan &) = delete;
@@ -124,3 +207,33 @@ struct Plan {
 };
 
 }  // namespace mtgp
+
+// A device generator set (mtgp_genset_t): two state slots per generator;
+// `cur` names the slot holding the current window.
+struct mtgp_genset {
+    mtgp::Context *ctx;
+    std::vector<int> bits, mexp, slot, n, cur;
+    std::vector<size_t> off;  // byte offset of slot pair in d_states
+    char *d_states = nullptr;
+    ~mtgp_genset() {
+        if (d_states)
+            (void)hipFree(d_states);
+    }
+    void *state(int g, int which) {
+        return d_states + off[(size_t)g] + (size_t)which * (size_t)n[(size_t)g] * (size_t)(bits[(size_t)g] / 8);
+    }
+    std::vector<mtgp::GenRef> refs() {
+        std::vector<mtgp::GenRef> r(bits.size());
+        for (size_t g = 0; g < bits.size(); g++) {
+            r[g].bits = bits[g];
+            r[g].mexp = mexp[g];
+            r[g].n = n[g];
+            r[g].slot = slot[g];
+            r[g].state_in = state((int)g, cur[g]);
+            r[g].state_out = state((int)g, cur[g] ^ 1);
+        }
+        return r;
+    }
+};
+
+

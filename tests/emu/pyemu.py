@@ -11,8 +11,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ITEM = np.dtype([("out", "<u8"), ("count", "<u8"), ("state_in", "<u8"), ("state_out", "<u8"),
                  ("poly", "<u8"), ("poly_words", "<i4"), ("param", "<i4"), ("kind", "<i4"),
-                 ("parts", "<i4"), ("part_bytes", "<u8")])
-assert ITEM.itemsize == 64
+                 ("parts", "<i4"), ("part_bytes", "<u8"), ("segs", "<u8"), ("nsegs", "<i4"), ("pad0", "<i4")])
+assert ITEM.itemsize == 80
+SEGMENT = np.dtype([("out", "<u8"), ("count", "<u8"), ("kind", "<i4"), ("pad0", "<i4"), ("pad1", "<u8")])
+assert SEGMENT.itemsize == 32
 
 
 def load():

@@ -1,0 +1,67 @@
+"""Plans captured in a hipGraph and replayed, and the autotuner."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _plan_case(mtgp, oracle, count=60000):
+    specs = [(32, 11213, 0), (32, 23209, 1), (64, 23209, 2), (32, 11213, 3)]
+    gs = mtgp.GenSet(specs)
+    seeds = [10, 20, 30, 40]
+    gs.init_seed(seeds)
+    reqs, want, bufs = [], [], []
+    for g, (bits, mexp, idx) in enumerate(specs):
+        p = oracle.params(bits, mexp, idx)
+        w = oracle.init_state(p, seeds[g])
+        for kind, skip in ((0, 0), (2, 777), (1, 0)):
+            _, w = oracle.generate(p, w, 0, skip) if skip else (None, w)
+            e, w = oracle.generate(p, w, kind, count)
+            buf = mtgp.DeviceBuffer(e.nbytes)
+            reqs.append((g, kind, count, skip, buf))
+            want.append(e)
+            bufs.append(buf)
+    return gs, reqs, want, bufs
+
+
+def _check(bufs, want, what):
+    for k, (b, e) in enumerate(zip(bufs, want)):
+        got = b.to_host(e.dtype, e.size)
+        assert (got.view(np.uint8) == e.view(np.uint8)).all(), "%s: request %d" % (what, k)
+
+
+def test_graph_replays_match_eager(mtgp, oracle):
+    """A captured plan (jump launch + generation launch) replayed three
+    times, with eager work and synchronisations between replays and the
+    outputs scribbled over before each replay; every replay must equal the
+    oracle (and so the eager launch)."""
+    gs, reqs, want, bufs = _plan_case(mtgp, oracle)
+    plan = gs.plan(reqs, segment=8192)
+    assert plan.stats()["items"] > 20
+    stream = mtgp.Stream()
+    plan.launch(stream)
+    stream.synchronize()
+    _check(bufs, want, "eager")
+    plan.capture(stream)
+    junk = np.full(max(e.nbytes for e in want), 0x5A, np.uint8)
+    other = mtgp.Generator32(11213, 9, seed=3)
+    for rep in range(3):
+        for b, e in zip(bufs, want):
+            b.from_host(junk[:e.nbytes])
+        other.fill(0, 50000)  # eager work between replays
+        mtgp.synchronize()
+        plan.launch_graph(stream)
+        stream.synchronize()
+        _check(bufs, want, "replay %d" % rep)
+
+
+def test_autotune_keeps_results_exact(mtgp, oracle):
+    """The autotuner tries every (layout, workgroups per CU) on the plan's
+    own buffers and keeps the fastest; the outputs after it are exact."""
+    gs, reqs, want, bufs = _plan_case(mtgp, oracle, count=200000)
+    plan = gs.plan(reqs, segment=16384)
+    layout, bpc, ms = plan.autotune(reps=1)
+    assert layout in (0, 1) and 1 <= bpc <= 32 and ms > 0
+    plan.launch()
+    mtgp.synchronize()
+    _check(bufs, want, "after autotune layout %d bpc %d" % (layout, bpc))

@@ -190,3 +190,56 @@ def test_no_write_past_requested_outputs(mtgp, oracle, layout, bits, mexp):
         got = host[offs[g]:offs[g] + c * eb].view(want.dtype)
         assert_same(got, want, "request %d" % g)
     assert (host[mask] == 0xA5).all(), "bytes outside the requested ranges were written"
+
+
+@pytest.mark.parametrize("jump_mode", ["0", "1"])
+def test_genset_chains_and_gaps(mtgp, oracle, monkeypatch, jump_mode):
+    """Many short requests per generator (chains of segments inside one
+    work item), short skips stepped over, long skips reached by a jump, a
+    2^40 skip, zero-count requests and a trailing skip, on a 4096-number
+    cell grid, against the oracle; both start-state jump modes."""
+    monkeypatch.setenv("MTGP_JUMP_MODE", jump_mode)
+    specs = [(32, 11213, 1), (64, 23209, 2), (32, 44497, 3)]
+    gs = mtgp.GenSet(specs)
+    seeds = [5, 6, 7]
+    gs.init_seed(seeds)
+    rng = np.random.default_rng(int(jump_mode) + 11)
+    state = {}
+    for g, (bits, mexp, idx) in enumerate(specs):
+        p = oracle.params(bits, mexp, idx)
+        words = mexp // 64 + 2
+        P = oracle.hex_to_poly(mtgp.charpoly_hex(bits, mexp, idx), words)
+        state[g] = [p, oracle.init_state(p, seeds[g]), P]
+    reqs, expect, bufs = [], [], []
+    for k in range(90):
+        g = k % 3
+        p, w, P = state[g]
+        kind = int(rng.integers(0, 5))
+        count = int(rng.integers(0, 3000)) if k % 17 else 0
+        r = rng.random()
+        skip = 0 if r < 0.7 else int(rng.integers(1, 500)) if r < 0.85 else int(rng.integers(70000, 200000)) \
+            if r < 0.97 else 1 << 40
+        if skip >= 1 << 30:
+            w = oracle.jump(p, w, oracle.poly_to_hex(oracle.xpow_mod(skip, P)))
+        elif skip:
+            _, w = oracle.generate(p, w, 0, skip)
+        want, w = oracle.generate(p, w, kind, count)
+        state[g][1] = w
+        dt = mtgp.out_dtype(specs[g][0], kind)
+        buf = mtgp.DeviceBuffer(max(1, count) * np.dtype(dt).itemsize)
+        bufs.append((buf, dt, count))
+        reqs.append((g, kind, count, skip, buf))
+        expect.append(want)
+    for g in range(3):  # trailing skip
+        p, w, P = state[g]
+        _, state[g][1] = oracle.generate(p, w, 0, 33)
+        reqs.append((g, 0, 0, 33, 0))
+    plan = gs.plan(reqs, segment=4096)
+    plan.launch()
+    mtgp.synchronize()
+    plan.commit()
+    for k, (buf, dt, count) in enumerate(bufs):
+        if count:
+            assert_same(buf.to_host(dt, count), expect[k], "request %d" % k)
+    for g in range(3):
+        assert_same(gs.state(g), state[g][1], "state %d" % g)

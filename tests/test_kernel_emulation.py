@@ -159,3 +159,38 @@ def test_item_start_window_from_parts(emu, oracle, bits, mexp):
                           n=p.n)], 1)
     e, we = oracle.generate(p, w, 0, n)
     assert (out == e).all() and (sout == we).all()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("bits,mexp", [(32, 11213), (32, 44497), (64, 23209), (64, 110503)])
+def test_item_segment_chain(emu, oracle, bits, mexp, layout):
+    """One item running a chain of segments: consecutive pieces of the
+    stream with different kinds, unaligned outputs and numbers stepped over
+    without output (out == 0), continuing one window from segment to
+    segment, then the final window."""
+    p = oracle.params(bits, mexp, 9)
+    eb = bits // 8
+    w0 = oracle.init_state(p, 77)
+    plan = [(0, 3), (2, 300), (None, 1000), (1, 257), (None, 5), (4, 2100), (3, 1), (0, 0), (2, 513)]
+    segs = np.zeros(len(plan), pyemu.SEGMENT)
+    keep, want, w = [], [], w0
+    for k, (kind, n) in enumerate(plan):
+        if kind is None:
+            segs[k] = (0, n, 0, 0, 0)
+            _, w = oracle.generate(p, w, 0, n)
+            continue
+        raw = np.zeros((n + 40) * eb + 256, np.uint8)
+        base = raw.ctypes.data + (-raw.ctypes.data) % 128 + eb * (5 * k % 16)
+        segs[k] = (base, n, kind, 0, 0)
+        e, w = oracle.generate(p, w, kind, n)
+        keep.append((raw, base, n, e))
+    sin, sout = w0.copy(), np.zeros_like(w0)
+    item = dict(state_in=sin.ctypes.data, state_out=sout.ctypes.data, param=pyemu.set_id(bits, mexp, 9),
+                segs=segs.ctypes.data, nsegs=len(plan), n=p.n)
+    _run(emu, bits, [item], layout)
+    for raw, base, n, e in keep:
+        start = base - raw.ctypes.data
+        got = raw[start:start + n * eb].view(e.dtype)
+        assert (_bits(got) == _bits(e)).all()
+        assert (raw[start + n * eb:] == 0).all() and (raw[:start] == 0).all()
+    assert (sout == w).all()
