@@ -278,6 +278,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--segment", type=int, default=0, help="numbers per segment (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-rate", type=float, default=20000.0, help="serving requests per second")
+    ap.add_argument("--serving-requests", type=int, default=12000)
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -319,6 +322,14 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(kern_gbs / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes": alg_bytes, "traffic": measured_traffic()},
     }
+    if not args.no_serving:
+        import mtgp_amd as M
+        sv = run_serving(M, args.serving_requests, args.serving_rate)
+        sv["workload"] = ("Poisson arrivals over 256 device generators (128 MTGP32-11213, 64 MTGP32-23209, "
+                          "64 MTGP64-23209); uint32 / float [1,2) / double [0,1) equally likely; 2^x numbers, "
+                          "x ~ N(16, 2.5) clipped to [10, 26]; 10% skip 2^U numbers first, U ~ U[10, 30]; "
+                          "4 GiB output budget")
+        line["serving"] = sv
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -874,10 +874,6 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
 // by all jumps of the batch, so LDS traffic per jump drops by the batch
 // size; the host supplies each jump's coefficients as per-residue masks.
 
-__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
-    return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
-}
-
 typedef GMEM uint64_t gu64c;
 
 // Lane `src`'s 64-bit value as a wave-uniform value.
@@ -1082,8 +1078,11 @@ __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restr
     }
 }
 
-// Steps per LDS relocation.
-constexpr int kK = 3;
+// Steps between window relocations (compile-time; -DMTGP_RELOC_STEPS=n).
+#ifndef MTGP_RELOC_STEPS
+#define MTGP_RELOC_STEPS 3
+#endif
+constexpr int kK = MTGP_RELOC_STEPS;
 
 template <int B, int NMAX, int LAYOUT>
 struct KernelOf;

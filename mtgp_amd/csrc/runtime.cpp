@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <functional>
 #include <map>
@@ -103,6 +104,11 @@ Context &context() {
     for (int b = 0; b < 2; b++)
         for (int cls = 0; cls < 3; cls++)
             c->bpc[b][cls] = c->default_bpc;
+    if (const char *e = getenv("MTGP_SEGMENT_SPLIT")) {
+        const double v = atof(e);
+        if (v >= 0 && v < 1)
+            c->segment_split = v;
+    }
     if (const char *e = getenv("MTGP_SEGMENT_ROUNDS")) {
         const double v = atof(e);
         if (v >= 0.25 && v <= 64)
@@ -663,6 +669,20 @@ void grow_pinned(char *&p, size_t &cap, size_t need) {
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 }  // namespace
 
+void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int slots, uint64_t &S1, uint64_t &S2,
+               double &frac) {
+    frac = c.segment_split;
+    if (frac <= 0 || frac >= 1) {
+        frac = 0;
+        S1 = S2 = auto_segment(c, total_numbers, n_requests, slots);
+        return;
+    }
+    const double usable = std::max(1.0, (double)slots - (double)n_requests);
+    auto round256 = [](double v) { return std::max<uint64_t>(((uint64_t)v + 255) & ~255ull, 4096); };
+    S1 = round256(std::ceil(frac * (double)total_numbers / usable));
+    S2 = round256(std::ceil((1 - frac) * (double)total_numbers / usable));
+}
+
 void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<mtgp_request_t> &reqs,
                  const PlanOptions &opt) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -689,7 +709,15 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         const int b = g.bits == 64, cls = kernel_class(g.bits, g.n);
         slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][c.layout[b][cls]]));
     }
-    const uint64_t S = opt.cache ? opt.cache->cell() : opt.segment ? opt.segment : auto_segment(c, all, nonempty, slots);
+    uint64_t S1 = 0, S2 = 0;
+    double frac = 0;
+    if (opt.cache)
+        S1 = S2 = opt.cache->cell();
+    else if (opt.segment)
+        S1 = S2 = opt.segment;
+    else
+        auto_grid(c, all, nonempty, slots, S1, S2, frac);
+    const uint64_t S = S1;
     // Requests as runs of stream positions per generator.
     struct Run {
         int req;
@@ -707,6 +735,20 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             runs[(size_t)r.gen].push_back(Run{ri, pos, r.count});
         pos += r.count;
     }
+    // Per generator: cells of S1 numbers below B[g] (a multiple of S1), of
+    // S2 numbers from B[g] on.
+    std::vector<uint64_t> Bg((size_t)G, UINT64_MAX);
+    if (frac > 0)
+        for (int g = 0; g < G; g++)
+            Bg[(size_t)g] = (uint64_t)(frac * (double)total[(size_t)g]) / S1 * S1;
+    auto cstart = [&](int g, uint64_t pos) {
+        const uint64_t B = Bg[(size_t)g];
+        return pos < B ? pos / S1 * S1 : B + (pos - B) / S2 * S2;
+    };
+    auto cend = [&](int g, uint64_t pos) {
+        const uint64_t B = Bg[(size_t)g];
+        return pos < B ? (pos / S1 + 1) * S1 : B + ((pos - B) / S2 + 1) * S2;
+    };
     // Cells.
     struct Piece {
         int req;  // -1: stepped over
@@ -727,14 +769,14 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         auto at = [&](uint64_t pos) {
             if (!cv.empty()) {
                 Cell &cur = cv.back();
-                if (pos / S == cur.start / S && pos >= cur.end && pos - cur.end <= opt.discard_max) {
+                if (cstart(g, pos) == cstart(g, cur.start) && pos >= cur.end && pos - cur.end <= opt.discard_max) {
                     if (pos > cur.end)
                         cur.pieces.push_back(Piece{-1, 0, pos - cur.end});
                     cur.end = pos;
                     return;
                 }
             }
-            const uint64_t g0 = pos / S * S;
+            const uint64_t g0 = cstart(g, pos);
             uint64_t start = (pos - g0 <= opt.discard_max) ? g0 : pos;
             if (!cv.empty() && start < cv.back().end)
                 start = pos;
@@ -749,7 +791,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             uint64_t done = 0;
             while (done < r.count) {
                 const uint64_t pos = r.pos + done;
-                const uint64_t take = std::min(r.count - done, (pos / S + 1) * S - pos);
+                const uint64_t take = std::min(r.count - done, cend(g, pos) - pos);
                 at(pos);
                 cv.back().pieces.push_back(Piece{r.req, done, take});
                 cv.back().end = pos + take;

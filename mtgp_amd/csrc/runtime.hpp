@@ -48,6 +48,13 @@ struct Context {
     // better, each extra segment costs one jump-ahead (1.5 measured best for
     // uneven MTGP32-11213 batches at 16 workgroups per CU).
     double segment_rounds = 1.5;
+    // Two-size automatic cells: the first `segment_split` of each
+    // generator's positions in cells sized for one round over all slots, the
+    // rest in cells sized for a second round (so the last round's cells are
+    // small and fill the tail); 0 = equal cells, segment_rounds rounds.
+    // 0.7 measured best on the batched MTGP32-11213 workload (generation
+    // 4.03-4.06 ms vs 4.21-4.25 ms with 1.5 rounds of equal cells).
+    double segment_split = 0.7;
     struct Custom {
         DevParam rec;
         unsigned char sha1[20];
@@ -93,6 +100,10 @@ struct GenRef {
 };
 
 uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, int slots);
+// Cell sizes for automatic segmentation: S1 for the first `frac` of each
+// generator's positions, S2 after (frac = 0: one size).
+void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int slots, uint64_t &S1, uint64_t &S2,
+               double &frac);
 
 // Jump polynomials x^pos mod P kept per parameter slot for positions on a
 // grid of `cell` numbers (pos = k * cell): table T1[k] = x^(k cell) for
