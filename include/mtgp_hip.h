@@ -42,6 +42,17 @@ enum {
     MTGP_KIND_OPEN_OPEN = 4     /* (0, 1) */
 };
 
+/* ------------------------------------------- dynamic-creator parameters
+ * The 200-set MEXP-11213 tables of the CUDA sample
+ * (cuda-sample/mtgp32dc-param-11213.c, mtgp64dc-param-11213.c), usable
+ * with the init family, the inline genrand family and mtgp*_fast_jump.
+ * MTGP32-DC sets also run on the GPU (mtgp32_fill, generator sets via
+ * mtgp_genset_create_params).  MTGP64-DC sets have N - pos < 256 (N = 176)
+ * and are host-only: the device step computes 256 words at a time, so the
+ * GPU entry points reject them with an error. */
+extern mtgp32_params_fast_t mtgp32dc_params_fast_11213[200];
+extern mtgp64_params_fast_t mtgp64dc_params_fast_11213[200];
+
 /* ------------------------------------------------------------ plumbing */
 const char *mtgp_last_error(void);
 /* Makes `device` current for this library (default: HIP's current device). */
@@ -95,6 +106,11 @@ int mtgp64_jump_hex(const mtgp64_params_fast_t *params, const uint64_t *steps, i
 typedef struct mtgp_genset mtgp_genset_t;
 
 mtgp_genset_t *mtgp_genset_create(int count, const int *bits, const int *mexp, const int *index);
+/* The same from parameter records: params[g] points to an
+ * mtgp32_params_fast_t (bits[g] == 32) or mtgp64_params_fast_t (64), e.g. a
+ * dynamic-creator set; a record that is not a table set must carry the
+ * poly_sha1 of its own characteristic polynomial (it is checked). */
+mtgp_genset_t *mtgp_genset_create_params(int count, const int *bits, const void *const *params);
 void mtgp_genset_destroy(mtgp_genset_t *set);
 int mtgp_genset_count(const mtgp_genset_t *set);
 /* Words in generator g's window (N). */

@@ -122,6 +122,7 @@ for b in (32, 64):
     _sig("mtgp%d_charpoly_hex" % b, i32, C.POINTER(P), cp, sz)
     _sig("mtgp%d_jump_hex" % b, i32, C.POINTER(P), C.POINTER(C.c_uint64), i32, cp, sz)
 _sig("mtgp_genset_create", vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("mtgp_genset_create_params", vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_void_p))
 _sig("mtgp_genset_destroy", None, vp)
 _sig("mtgp_genset_count", i32, vp)
 _sig("mtgp_genset_state_words", i32, vp, i32)
@@ -183,6 +184,15 @@ def params(bits, mexp, index):
     P = Params32 if bits == 32 else Params64
     table = (P * 128).in_dll(_lib, "mtgp%d_params_fast_%d" % (bits, mexp))
     return table[index]
+
+
+def dc_params(bits, index):
+    """The dynamic-creator record mtgp{bits}dc_params_fast_11213[index]
+    (200 sets; cuda-sample/mtgp{bits}dc-param-11213.c)."""
+    if bits not in (32, 64) or not 0 <= index < 200:
+        raise ValueError("no dynamic-creator set mtgp%d-11213 #%d" % (bits, index))
+    P = Params32 if bits == 32 else Params64
+    return (P * 200).in_dll(_lib, "mtgp%ddc_params_fast_11213" % bits)[index]
 
 
 def _wdtype(bits):
@@ -294,8 +304,9 @@ class Event:
 class _Generator:
     bits = 32
 
-    def __init__(self, mexp, index, seed=None, array=None, string=None):
-        self.p = params(self.bits, mexp, index)
+    def __init__(self, mexp=None, index=None, seed=None, array=None, string=None, record=None):
+        """A table set (mexp, index), or any parameter record (`record`)."""
+        self.p = record if record is not None else params(self.bits, mexp, index)
         F = Fast32 if self.bits == 32 else Fast64
         self.g = F()
         W = C.c_uint32 if self.bits == 32 else C.c_uint64
@@ -359,19 +370,21 @@ class Generator64(_Generator):
     bits = 64
 
 
-def charpoly_hex(bits, mexp, index):
-    p = params(bits, mexp, index)
-    buf = C.create_string_buffer(mexp // 4 + 8)
+def charpoly_hex(bits, mexp=None, index=None, record=None):
+    """Characteristic polynomial of a table set or of a record, as hex."""
+    p = record if record is not None else params(bits, mexp, index)
+    buf = C.create_string_buffer(p.mexp // 4 + 8)
     _check(getattr(_lib, "mtgp%d_charpoly_hex" % bits)(C.byref(p), buf, len(buf)), "charpoly")
     return buf.value.decode()
 
 
-def jump_hex(bits, mexp, index, steps):
-    """x^steps mod P as a jump string (steps: Python int >= 0)."""
-    p = params(bits, mexp, index)
+def jump_hex(bits, mexp=None, index=None, steps=0, record=None):
+    """x^steps mod P as a jump string (steps: Python int >= 0), for a table
+    set or a record."""
+    p = record if record is not None else params(bits, mexp, index)
     words = max(1, (int(steps).bit_length() + 63) // 64)
     e = (C.c_uint64 * words)(*[(int(steps) >> (64 * i)) & (2**64 - 1) for i in range(words)])
-    buf = C.create_string_buffer(mexp // 4 + 8)
+    buf = C.create_string_buffer(p.mexp // 4 + 8)
     _check(getattr(_lib, "mtgp%d_jump_hex" % bits)(C.byref(p), e, words, buf, len(buf)), "jump_hex")
     return buf.value.decode()
 
@@ -381,13 +394,21 @@ class GenSet:
     """Device-resident generators (mtgp_genset_t)."""
 
     def __init__(self, specs):
-        """specs: list of (bits, mexp, index)."""
-        self.specs = [tuple(int(v) for v in s) for s in specs]
-        n = len(self.specs)
-        b = (C.c_int * n)(*[s[0] for s in self.specs])
-        m = (C.c_int * n)(*[s[1] for s in self.specs])
-        i = (C.c_int * n)(*[s[2] for s in self.specs])
-        self.h = _lib.mtgp_genset_create(n, b, m, i)
+        """specs: list of (bits, mexp, index) table sets, or of parameter
+        records (Params32 / Params64 instances, e.g. dc_params(32, i))."""
+        n = len(specs)
+        if all(isinstance(s, (Params32, Params64)) for s in specs):
+            self._recs = list(specs)
+            self.specs = [(32 if isinstance(s, Params32) else 64, s.mexp, -1) for s in specs]
+            b = (C.c_int * n)(*[s[0] for s in self.specs])
+            ptrs = (C.c_void_p * n)(*[C.cast(C.pointer(r), C.c_void_p) for r in self._recs])
+            self.h = _lib.mtgp_genset_create_params(n, b, ptrs)
+        else:
+            self.specs = [tuple(int(v) for v in s) for s in specs]
+            b = (C.c_int * n)(*[s[0] for s in self.specs])
+            m = (C.c_int * n)(*[s[1] for s in self.specs])
+            i = (C.c_int * n)(*[s[2] for s in self.specs])
+            self.h = _lib.mtgp_genset_create(n, b, m, i)
         if not self.h:
             raise MTGPError("genset_create: " + last_error())
 

@@ -101,12 +101,33 @@ static_assert(sizeof(all64) / sizeof(all64[0]) == 384, "mtgp64 sets");
 static_assert(all32[0].mexp == 11213 && all32[128].mexp == 23209 && all32[256].mexp == 44497, "order");
 static_assert(all64[0].mexp == 23209 && all64[128].mexp == 44497 && all64[256].mexp == 110503, "order");
 
+// Dynamic-creator sets (cuda-sample/mtgp32dc-param-11213.c,
+// mtgp64dc-param-11213.c), same record form.
+#define MTGP32DC_SET(m, i, p, s1, s2, a0, a1, a2, a3, t0, t1, t2, t3, sha) \
+    make32(m, p, s1, s2, a0, a1, a2, a3, t0, t1, t2, t3, sha),
+#define MTGP64DC_SET(m, i, p, s1, s2, a0, a1, a2, a3, t0, t1, t2, t3, sha)
+constexpr mtgp32_params_fast_t dc32[] = {
+#include "dc_param_sets.inc"
+};
+#undef MTGP32DC_SET
+#undef MTGP64DC_SET
+#define MTGP32DC_SET(m, i, p, s1, s2, a0, a1, a2, a3, t0, t1, t2, t3, sha)
+#define MTGP64DC_SET(m, i, p, s1, s2, a0, a1, a2, a3, t0, t1, t2, t3, sha) \
+    make64(m, p, s1, s2, a0, a1, a2, a3, t0, t1, t2, t3, sha),
+constexpr mtgp64_params_fast_t dc64[] = {
+#include "dc_param_sets.inc"
+};
+#undef MTGP32DC_SET
+#undef MTGP64DC_SET
+static_assert(sizeof(dc32) / sizeof(dc32[0]) == 200 && sizeof(dc64) / sizeof(dc64[0]) == 200, "dc sets");
+
 }  // namespace
 
 #define R4(T, b) T[(b)], T[(b) + 1], T[(b) + 2], T[(b) + 3]
 #define R16(T, b) R4(T, b), R4(T, (b) + 4), R4(T, (b) + 8), R4(T, (b) + 12)
 #define R64(T, b) R16(T, b), R16(T, (b) + 16), R16(T, (b) + 32), R16(T, (b) + 48)
 #define R128(T, b) R64(T, b), R64(T, (b) + 64)
+#define R200(T) R128(T, 0), R64(T, 128), R4(T, 192), R4(T, 196)
 
 extern "C" {
 mtgp32_params_fast_t mtgp32_params_fast_11213[128] = {R128(all32, 0)};
@@ -115,6 +136,8 @@ mtgp32_params_fast_t mtgp32_params_fast_44497[128] = {R128(all32, 256)};
 mtgp64_params_fast_t mtgp64_params_fast_23209[128] = {R128(all64, 0)};
 mtgp64_params_fast_t mtgp64_params_fast_44497[128] = {R128(all64, 128)};
 mtgp64_params_fast_t mtgp64_params_fast_110503[128] = {R128(all64, 256)};
+mtgp32_params_fast_t mtgp32dc_params_fast_11213[200] = {R200(dc32)};
+mtgp64_params_fast_t mtgp64dc_params_fast_11213[200] = {R200(dc64)};
 }
 
 namespace mtgp {

@@ -73,7 +73,7 @@ Context &context() {
     c->hbm_bytes = prop.totalGlobalMem;
     c->arch = prop.gcnArchName;
     // Parameter table: the 768 reference sets, then room for custom sets.
-    c->param_capacity = kNumSets + 256;
+    c->param_capacity = kNumSets + 1024;
     hip_check(hipMalloc((void **)&c->d_params, sizeof(DevParam) * (size_t)c->param_capacity), "hipMalloc params");
     std::vector<DevParam> host(kNumSets);
     for (int k = 0; k < kNumSets; k++)
@@ -170,6 +170,12 @@ size_t Context::lds_reserve(const void *fn, int per_cu) {
 // Parameter slot of an arbitrary (host) parameter record: a table set when
 // it equals one, else a custom slot uploaded on first use.
 int Context::slot_of(const DevParam &rec, const unsigned char *sha1) {
+    // The device step computes 256 words at once, which needs N - pos >= 256
+    // (every table set has it; the MTGP64-DC sets, N = 176, do not).
+    if (rec.n - rec.pos < 256)
+        throw Error("parameter set with N - pos = " + std::to_string(rec.n - rec.pos) +
+                    " < 256 cannot run on the device (MTGP" + std::to_string(rec.bits) + "-" +
+                    std::to_string(rec.mexp) + ", pos " + std::to_string(rec.pos) + ")");
     for (int k = 0; k < kNumSets; k++)
         if (memcmp(&host_param(k), &rec, sizeof(DevParam)) == 0)
             return k;
@@ -1564,8 +1570,9 @@ int mtgp_event_elapsed_ms(void *a, void *b, float *ms) {
 
 // ------------------------------------------------------------ gensets
 
-mtgp_genset_t *mtgp_genset_create(int count, const int *bits, const int *mexp, const int *index) {
-    API_BEGIN
+// known_slots: table slots when the records come from the tables.
+static mtgp_genset_t *genset_of(int count, const int *bits, const std::vector<const void *> &recs,
+                                const std::vector<int> *known_slots = nullptr) {
     if (count <= 0)
         throw Error("generator count must be positive");
     Context &c = context();
@@ -1573,14 +1580,29 @@ mtgp_genset_t *mtgp_genset_create(int count, const int *bits, const int *mexp, c
     s->ctx = &c;
     size_t bytes = 0;
     for (int g = 0; g < count; g++) {
-        const int slot = global_set_id(bits[g], mexp[g], index[g]);
-        if (slot < 0)
-            throw Error("no parameter set mtgp" + std::to_string(bits[g]) + "-" + std::to_string(mexp[g]) + " #" +
-                        std::to_string(index[g]));
+        int slot, mexp;
+        if (bits[g] == 32) {
+            const auto &p = *static_cast<const mtgp32_params_fast_t *>(recs[(size_t)g]);
+            slot = known_slots ? (*known_slots)[(size_t)g] : c.slot_of(dev_param_of(p), p.poly_sha1);
+            mexp = p.mexp;
+            s->p32.push_back(p);
+            s->p64.emplace_back();
+        } else if (bits[g] == 64) {
+            const auto &p = *static_cast<const mtgp64_params_fast_t *>(recs[(size_t)g]);
+            check64(p);
+            slot = known_slots ? (*known_slots)[(size_t)g] : c.slot_of(dev_param_of(p), p.poly_sha1);
+            mexp = p.mexp;
+            s->p64.push_back(p);
+            s->p32.emplace_back();
+        } else {
+            throw Error("word size must be 32 or 64");
+        }
+        if (kernel_class(bits[g], state_words(bits[g], mexp)) < 0)
+            throw Error("state of MEXP " + std::to_string(mexp) + " is larger than any kernel class");
         s->bits.push_back(bits[g]);
-        s->mexp.push_back(mexp[g]);
+        s->mexp.push_back(mexp);
         s->slot.push_back(slot);
-        s->n.push_back(state_words(bits[g], mexp[g]));
+        s->n.push_back(state_words(bits[g], mexp));
         s->cur.push_back(0);
         s->off.push_back(bytes);
         bytes += 2 * (size_t)s->n.back() * (size_t)(bits[g] / 8);
@@ -1589,6 +1611,35 @@ mtgp_genset_t *mtgp_genset_create(int count, const int *bits, const int *mexp, c
     hip_check(hipMalloc((void **)&s->d_states, bytes), "hipMalloc states");
     hip_check(hipMemset(s->d_states, 0, bytes), "hipMemset states");
     return s.release();
+}
+
+mtgp_genset_t *mtgp_genset_create(int count, const int *bits, const int *mexp, const int *index) {
+    API_BEGIN
+    if (count <= 0)
+        throw Error("generator count must be positive");
+    std::vector<const void *> recs;
+    std::vector<int> slots;
+    for (int g = 0; g < count; g++) {
+        const int t = table_of(bits[g], mexp[g]);
+        if (t < 0 || index[g] < 0 || index[g] >= kSetsPerTable)
+            throw Error("no parameter set mtgp" + std::to_string(bits[g]) + "-" + std::to_string(mexp[g]) + " #" +
+                        std::to_string(index[g]));
+        static mtgp32_params_fast_t *const t32[3] = {mtgp32_params_fast_11213, mtgp32_params_fast_23209,
+                                                     mtgp32_params_fast_44497};
+        static mtgp64_params_fast_t *const t64[3] = {mtgp64_params_fast_23209, mtgp64_params_fast_44497,
+                                                     mtgp64_params_fast_110503};
+        recs.push_back(t < 3 ? (const void *)&t32[t][index[g]] : (const void *)&t64[t - 3][index[g]]);
+        slots.push_back(global_set_id(bits[g], mexp[g], index[g]));
+    }
+    return genset_of(count, bits, recs, &slots);
+    API_END(nullptr)
+}
+
+mtgp_genset_t *mtgp_genset_create_params(int count, const int *bits, const void *const *params) {
+    API_BEGIN
+    if (count <= 0)
+        throw Error("generator count must be positive");
+    return genset_of(count, bits, std::vector<const void *>(params, params + count));
     API_END(nullptr)
 }
 
@@ -1614,18 +1665,13 @@ static void upload_window(mtgp_genset *s, int g, const void *w) {
 int mtgp_genset_init_seed(mtgp_genset_t *s, const uint64_t *seeds) {
     API_BEGIN
     for (int g = 0; g < (int)s->bits.size(); g++) {
-        const int t = s->slot[(size_t)g] / kSetsPerTable, i = s->slot[(size_t)g] % kSetsPerTable;
         if (s->bits[(size_t)g] == 32) {
-            static mtgp32_params_fast_t *const t32[3] = {mtgp32_params_fast_11213, mtgp32_params_fast_23209,
-                                                         mtgp32_params_fast_44497};
             std::vector<uint32_t> w((size_t)s->n[(size_t)g]);
-            mtgp32_init_state(w.data(), &t32[t][i], (uint32_t)seeds[g]);
+            mtgp32_init_state(w.data(), &s->p32[(size_t)g], (uint32_t)seeds[g]);
             upload_window(s, g, w.data());
         } else {
-            static mtgp64_params_fast_t *const t64[3] = {mtgp64_params_fast_23209, mtgp64_params_fast_44497,
-                                                         mtgp64_params_fast_110503};
             std::vector<uint64_t> w((size_t)s->n[(size_t)g]);
-            mtgp64_init_state(w.data(), &t64[t - 3][i], seeds[g]);
+            mtgp64_init_state(w.data(), &s->p64[(size_t)g], seeds[g]);
             upload_window(s, g, w.data());
         }
     }
@@ -1636,25 +1682,20 @@ int mtgp_genset_init_seed(mtgp_genset_t *s, const uint64_t *seeds) {
 int mtgp_genset_init_array(mtgp_genset_t *s, const uint64_t *keys, const int *offsets, const int *lengths) {
     API_BEGIN
     for (int g = 0; g < (int)s->bits.size(); g++) {
-        const int t = s->slot[(size_t)g] / kSetsPerTable, i = s->slot[(size_t)g] % kSetsPerTable;
         const uint64_t *k = keys + offsets[g];
         if (s->bits[(size_t)g] == 32) {
-            static mtgp32_params_fast_t *const t32[3] = {mtgp32_params_fast_11213, mtgp32_params_fast_23209,
-                                                         mtgp32_params_fast_44497};
             std::vector<uint32_t> key((size_t)lengths[g]);
             for (int j = 0; j < lengths[g]; j++)
                 key[(size_t)j] = (uint32_t)k[j];
             mtgp32_fast_t tmp;
-            if (mtgp32_init_by_array(&tmp, &t32[t][i], key.data(), lengths[g]))
+            if (mtgp32_init_by_array(&tmp, &s->p32[(size_t)g], key.data(), lengths[g]))
                 throw Error("out of host memory");
             upload_window(s, g, tmp.status->array);
             mtgp32_free(&tmp);
         } else {
-            static mtgp64_params_fast_t *const t64[3] = {mtgp64_params_fast_23209, mtgp64_params_fast_44497,
-                                                         mtgp64_params_fast_110503};
             std::vector<uint64_t> key(k, k + lengths[g]);
             mtgp64_fast_t tmp;
-            if (mtgp64_init_by_array(&tmp, &t64[t - 3][i], key.data(), lengths[g]))
+            if (mtgp64_init_by_array(&tmp, &s->p64[(size_t)g], key.data(), lengths[g]))
                 throw Error("out of host memory");
             upload_window(s, g, tmp.status->array);
             mtgp64_free(&tmp);

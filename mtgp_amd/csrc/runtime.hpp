@@ -224,6 +224,9 @@ struct Plan {
 struct mtgp_genset {
     mtgp::Context *ctx;
     std::vector<int> bits, mexp, slot, n, cur;
+    // Parameter record of each generator (the one of its word size is used).
+    std::vector<mtgp32_params_fast_t> p32;
+    std::vector<mtgp64_params_fast_t> p64;
     std::vector<size_t> off;  // byte offset of slot pair in d_states
     char *d_states = nullptr;
     ~mtgp_genset() {

@@ -70,6 +70,22 @@ class Oracle:
             raise ValueError("no parameter set %d/%d/%d" % (bits, mexp, index))
         return p
 
+    @staticmethod
+    def params_from_record(bits, rec, index=-1):
+        """Params from a 73-value record laid out as mtgp*-fast.h:47-58
+        (mexp, pos, sh1, sh2, tbl[16], tmp_tbl[16], flt/dbl table[16], mask,
+        poly_sha1[20]) -- e.g. one the reference library wrote."""
+        rec = [int(v) for v in rec]
+        p = Params()
+        p.bits, p.mexp, p.index = bits, rec[0], index
+        p.n = rec[0] // bits + 1
+        p.pos, p.sh1, p.sh2 = rec[1], rec[2], rec[3]
+        for j in range(16):
+            p.tbl[j], p.tmp[j], p.flt[j] = rec[4 + j], rec[20 + j], rec[36 + j]
+        p.mask = rec[52]
+        p.sha1 = bytes(rec[53:73]).hex().encode()
+        return p
+
     def _wdtype(self, bits):
         return np.uint32 if bits == 32 else np.uint64
 
@@ -163,7 +179,7 @@ class RefLib:
         L.rs_fill.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_void_p]
         L.rs_window.argtypes = [C.c_void_p, C.c_void_p]
         L.rs_idstring.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
-        if variant == "fast":
+        if variant in ("fast", "dc"):
             L.rs_set_window.argtypes = [C.c_void_p, C.c_void_p]
             L.rs_jump.argtypes = [C.c_void_p, C.c_char_p]
             L.rs_init_state.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_void_p]

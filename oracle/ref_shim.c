@@ -9,6 +9,8 @@
  *   SHIM_FAST64  mtgp64-fast.{h,c}, mtgp64-param-fast.c, mtgp64-fast-jump.c
  *   SHIM_REF32   mtgp32-ref.{h,c}, mtgp32-param-ref.c
  *   SHIM_REF64   mtgp64-ref.{h,c}, mtgp64-param-ref.c
+ * With SHIM_DC as well (fast variants), mexp 11213 selects the 200-set
+ * dynamic-creator table (cuda-sample/mtgp*dc-param-11213.c) instead.
  * The genrand family is `inline static` in the fast headers, so a bulk
  * "fill" loop here is the only way to call it from outside C.
  *
@@ -58,6 +60,21 @@ typedef mtgp64_params_ref_t par_t;
 #error "select a shim variant"
 #endif
 
+#if defined(SHIM_DC)
+#if W == 32
+extern par_t mtgp32dc_params_fast_11213[];
+#define DCTBL mtgp32dc_params_fast_11213
+#else
+extern par_t mtgp64dc_params_fast_11213[];
+#define DCTBL mtgp64dc_params_fast_11213
+#endif
+static const par_t *lookup(int mexp, int index)
+{
+    if (mexp != 11213 || index < 0 || index > 199)
+        return NULL;
+    return DCTBL + index;
+}
+#else
 static const par_t *lookup(int mexp, int index)
 {
     const par_t *t = TBL(mexp);
@@ -65,6 +82,7 @@ static const par_t *lookup(int mexp, int index)
         return NULL;
     return t + index;
 }
+#endif
 
 void *rs_new_seed(int mexp, int index, uint64_t seed)
 {

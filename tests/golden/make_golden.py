@@ -19,6 +19,12 @@ Files written (all small, loaded with numpy allow_pickle=False or json):
   reference_jump_data.npz  CHAR32_11213, JUMP3_162, mtgp32_jump_table[6][351]
   streams.json          SHA-256 digests of the benchmark-shaped streams
   substreams.npz        substreams i*3^162 of set 0, seed 777
+  dc_params_digests.json  the same for the dynamic-creator tables (200+200)
+  dc_outputs.npz        test-program sequences and stepped outputs of
+                        dynamic-creator sets (reference built with
+                        cuda-sample/mtgp*dc-param-11213.c)
+
+Steps can be selected by name: make_golden.py dc_golden
 """
 import hashlib
 import json
@@ -194,7 +200,45 @@ def substreams():
     np.savez_compressed(os.path.join(HERE, "substreams.npz"), **arrays)
 
 
+DC_SETS = {32: (0, 16, 199), 64: (0, 172, 199)}  # 16 and 172 have pos = 3
+
+
+def dc_golden():
+    digests = {"digests": {}, "records": {}}
+    arrays = {}
+    for bits in (32, 64):
+        ref = RefLib("dc", bits)
+        ds = []
+        for i in range(200):
+            rec = ref.params(11213, i)
+            ds.append(sha(rec))
+            if i in (0, 199):
+                digests["records"]["%d/%d" % (bits, i)] = [int(v) for v in rec]
+        digests["digests"]["%d" % bits] = ds
+        n = 11213 // bits + 1
+        for idx in DC_SETS[bits]:
+            key = "%d_%d" % (bits, idx)
+            g = ref.new_seed(11213, idx, 1)
+            arrays[key + "_init"] = g.fill(0, 1000)
+            for kind in (1, 2, 3, 4):
+                arrays[key + "_init_k%d" % kind] = g.fill(kind, 1000)
+            arrays[key + "_array"] = ref.new_array(11213, idx, [1, 2, 3, 4]).fill(0, 1000)
+            for s in (1, n + 1, 3003, 200004):
+                g = ref.new_seed(11213, idx, 0)
+                g.fill(0, 3)
+                g.fill(0, s)
+                arrays[key + "_step_%d" % s] = g.fill(0, 100)
+    with open(os.path.join(HERE, "dc_params_digests.json"), "w") as f:
+        json.dump(digests, f)
+    np.savez_compressed(os.path.join(HERE, "dc_outputs.npz"), **arrays)
+
+
+STEPS = (params_digests, main_outputs, jump_steps, reference_jump_data, substreams, streams, dc_golden)
+
 if __name__ == "__main__":
-    for step in (params_digests, main_outputs, jump_steps, reference_jump_data, substreams, streams):
+    wanted = sys.argv[1:]
+    for step in STEPS:
+        if wanted and step.__name__ not in wanted:
+            continue
         print("writing", step.__name__, flush=True)
         step()

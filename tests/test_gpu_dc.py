@@ -1,0 +1,66 @@
+"""Dynamic-creator MTGP32 sets on the GPU, against the reference's outputs
+(tests/golden/dc_outputs.npz); MTGP64-DC sets are rejected on the device."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def arrays():
+    return np.load(os.path.join(GOLD, "dc_outputs.npz"))
+
+
+@pytest.mark.parametrize("idx", [0, 16, 199])
+def test_dc32_fill_and_jump(mtgp, arrays, idx):
+    rec = mtgp.dc_params(32, idx)
+    key = "32_%d" % idx
+    g = mtgp.Generator32(record=rec, seed=1)
+    assert (g.fill(0, 1000) == arrays[key + "_init"]).all()
+    for kind in (1, 2, 3, 4):
+        got = g.fill(kind, 1000)
+        assert (got.view(np.uint8) == arrays[key + "_init_k%d" % kind].view(np.uint8)).all(), kind
+    g = mtgp.Generator32(record=rec, array=[1, 2, 3, 4])
+    assert (g.fill(0, 1000) == arrays[key + "_array"]).all()
+    for s in (1, 352, 3003, 200004):
+        g = mtgp.Generator32(record=rec, seed=0)
+        g.fill(0, 3)
+        g.fast_jump(mtgp.jump_hex(32, steps=s, record=rec))
+        assert (g.fill(0, 100) == arrays["%s_step_%d" % (key, s)]).all(), s
+
+
+def test_dc32_genset_plan(mtgp, oracle, arrays):
+    """A generator set built from DC records, each stream segmented across
+    the GPU by jump-ahead (jump polynomials from the records' own
+    characteristic polynomials): the reference's head and the oracle's
+    full stream."""
+    from pyoracle import Oracle
+    from test_dc import _record
+    idxs = (0, 16, 199)
+    recs = [mtgp.dc_params(32, i) for i in idxs]
+    gs = mtgp.GenSet(recs)
+    gs.init_seed([1, 1, 1])
+    n = 300000
+    bufs = [mtgp.DeviceBuffer(4 * n) for _ in recs]
+    plan = gs.plan([(g, 0, n, 0, bufs[g]) for g in range(3)], segment=16384)
+    assert plan.stats()["items"] > 30
+    plan.launch()
+    mtgp.synchronize()
+    for g, idx in enumerate(idxs):
+        got = bufs[g].to_host(np.uint32, n)
+        assert (got[:1000] == arrays["32_%d_init" % idx]).all(), idx
+        p = Oracle.params_from_record(32, _record(recs[g]))
+        want, _ = oracle.generate(p, oracle.init_state(p, 1), 0, n)
+        assert (got == want).all(), idx
+
+
+def test_dc64_rejected_on_device(mtgp):
+    g = mtgp.Generator64(record=mtgp.dc_params(64, 0), seed=1)
+    with pytest.raises(mtgp.MTGPError, match="N - pos"):
+        g.fill(0, 100)
+    with pytest.raises(mtgp.MTGPError, match="N - pos"):
+        mtgp.GenSet([mtgp.dc_params(64, 0)])
