@@ -66,6 +66,10 @@ int mtgp_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 /* Device-to-device copy by a plain streaming kernel (enqueued on `stream`,
  * NULL = default stream).  Reference point for hardware byte counters. */
 int mtgp_device_copy(void *dst, const void *src, size_t bytes, void *stream);
+/* Fills `bytes` (a multiple of 4) at dst with a 32-bit value by a plain
+ * streaming-store kernel (kernel nodes only, so it can be captured in a
+ * graph); also the reference point for the HBM write ceiling. */
+int mtgp_device_fill(void *dst, uint32_t value, size_t bytes, void *stream);
 int mtgp_synchronize(void);
 /* Streams and events for callers without their own HIP code (timing). */
 void *mtgp_stream_create(void);
@@ -121,6 +125,12 @@ int mtgp_genset_init_seed(mtgp_genset_t *set, const uint64_t *seeds);
  * (low 32 bits of each key word for mtgp32). */
 int mtgp_genset_init_array(mtgp_genset_t *set, const uint64_t *keys, const int *offsets,
                            const int *lengths);
+/* The same two, computed on the GPU (one wave per generator, window built
+ * in LDS and written to the set's current slot): the same windows, without
+ * host work or uploads for large sets.  Synchronous on `stream`. */
+int mtgp_genset_init_seed_device(mtgp_genset_t *set, const uint64_t *seeds, void *stream);
+int mtgp_genset_init_array_device(mtgp_genset_t *set, const uint64_t *keys, const int *offsets,
+                                  const int *lengths, void *stream);
 /* Host <-> device copy of one window (uint32_t or uint64_t words). */
 int mtgp_genset_set_state(mtgp_genset_t *set, int g, const void *window);
 int mtgp_genset_get_state(mtgp_genset_t *set, int g, void *window);

@@ -101,6 +101,7 @@ _sig("mtgp_dev_free", i32, vp)
 _sig("mtgp_memcpy_htod", i32, vp, vp, sz)
 _sig("mtgp_memcpy_dtoh", i32, vp, vp, sz)
 _sig("mtgp_device_copy", i32, vp, vp, sz, vp)
+_sig("mtgp_device_fill", i32, vp, C.c_uint32, sz, vp)
 _sig("mtgp_synchronize", i32)
 _sig("mtgp_stream_create", vp)
 _sig("mtgp_stream_destroy", i32, vp)
@@ -128,6 +129,8 @@ _sig("mtgp_genset_count", i32, vp)
 _sig("mtgp_genset_state_words", i32, vp, i32)
 _sig("mtgp_genset_init_seed", i32, vp, C.POINTER(C.c_uint64))
 _sig("mtgp_genset_init_array", i32, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("mtgp_genset_init_seed_device", i32, vp, C.POINTER(C.c_uint64), vp)
+_sig("mtgp_genset_init_array_device", i32, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_int), vp)
 _sig("mtgp_genset_set_state", i32, vp, i32, vp)
 _sig("mtgp_genset_get_state", i32, vp, i32, vp)
 _sig("mtgp_genset_jump", i32, vp, i32, cp)
@@ -256,6 +259,13 @@ def memcpy_dtoh(host, device_address):
     """Copies host.nbytes bytes from a device address into a host array."""
     _check(_lib.mtgp_memcpy_dtoh(host.ctypes.data, device_address, host.nbytes), "memcpy dtoh")
     return host
+
+
+def device_fill(dst, value, nbytes, stream=None):
+    """Fills nbytes of a DeviceBuffer / device address with a 32-bit value."""
+    d = dst.ptr if isinstance(dst, DeviceBuffer) else dst
+    _check(_lib.mtgp_device_fill(d, int(value) & 0xFFFFFFFF, int(nbytes), stream.handle if stream else None),
+           "device_fill")
 
 
 def device_copy(dst, src, nbytes, stream=None):
@@ -420,19 +430,28 @@ class GenSet:
     def __len__(self):
         return len(self.specs)
 
-    def init_seed(self, seeds):
+    def init_seed(self, seeds, device=False, stream=None):
+        """mtgp*_init for every generator (device=True: on the GPU)."""
         s = (C.c_uint64 * len(self))(*[int(v) & (2**64 - 1) for v in seeds])
-        _check(_lib.mtgp_genset_init_seed(self.h, s), "init_seed")
+        if device:
+            _check(_lib.mtgp_genset_init_seed_device(self.h, s, stream.handle if stream else None), "init_seed")
+        else:
+            _check(_lib.mtgp_genset_init_seed(self.h, s), "init_seed")
 
-    def init_array(self, keys):
+    def init_array(self, keys, device=False, stream=None):
+        """mtgp*_init_by_array for every generator (device=True: on the GPU)."""
         flat, offs, lens = [], [], []
         for k in keys:
             offs.append(len(flat))
             lens.append(len(k))
             flat.extend(int(v) & (2**64 - 1) for v in k)
         n = len(self)
-        _check(_lib.mtgp_genset_init_array(self.h, (C.c_uint64 * max(1, len(flat)))(*flat),
-                                           (C.c_int * n)(*offs), (C.c_int * n)(*lens)), "init_array")
+        args = ((C.c_uint64 * max(1, len(flat)))(*flat), (C.c_int * n)(*offs), (C.c_int * n)(*lens))
+        if device:
+            _check(_lib.mtgp_genset_init_array_device(self.h, *args, stream.handle if stream else None),
+                   "init_array")
+        else:
+            _check(_lib.mtgp_genset_init_array(self.h, *args), "init_array")
 
     def state(self, g):
         n = _lib.mtgp_genset_state_words(self.h, g)

@@ -70,6 +70,16 @@ struct alignas(16) JumpItem {
 };
 static_assert(sizeof(JumpItem) == 64, "JumpItem layout");
 
+// Device initialisation of one generator's window (mtgp*_init_state or
+// mtgp*_init_by_array computed on the GPU).
+struct alignas(16) InitItem {
+    uint64_t state;    // window out: N words (uint32_t or uint64_t)
+    uint64_t seed;     // mode 0: the seed; mode 1: address of key_len uint64_t keys
+    uint64_t hidden;   // tbl[4] ^ (tbl[8] << 16) of the parameter set
+    int32_t n, bits, key_len, mode;
+};
+static_assert(sizeof(InitItem) == 48, "InitItem layout");
+
 // Expanded parameter record as the kernels read it.  For MTGP64 the
 // tables hold the high 32 bits of each entry (all low halves are zero).
 struct alignas(16) DevParam {

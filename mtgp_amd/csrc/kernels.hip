@@ -1168,6 +1168,30 @@ __global__ __launch_bounds__(256) void copy16_kernel(uint64_t src_addr, uint64_t
 }
 }  // namespace
 
+namespace {
+// Plain streaming stores: each single-wave workgroup writes its own
+// contiguous rows of 256 words, one 256-byte dword store per instruction
+// (the store pattern of the generation kernels, without the generator).
+__global__ __launch_bounds__(64) void fill_rows_kernel(uint64_t dst_addr, uint64_t n_words, uint64_t per_wave,
+                                                       uint32_t value) {
+    gu32 *dst = reinterpret_cast<gu32 *>(dst_addr);
+    const uint64_t begin = (uint64_t)blockIdx.x * per_wave;
+    const uint64_t end = begin + per_wave < n_words ? begin + per_wave : n_words;
+    for (uint64_t k = begin + threadIdx.x; k < end; k += kWave)
+        dst[k] = value;
+}
+}  // namespace
+
+hipError_t launch_fill(void *dst, uint64_t n_words, uint32_t value, int waves, hipStream_t s) {
+    if (n_words == 0)
+        return hipSuccess;
+    const uint64_t per = ((n_words + (uint64_t)waves - 1) / (uint64_t)waves + 255) & ~(uint64_t)255;
+    const int grid = (int)((n_words + per - 1) / per);
+    hipLaunchKernelGGL(fill_rows_kernel, dim3(grid), dim3(kWave), 0, s, (uint64_t)(uintptr_t)dst, n_words, per,
+                       value);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, int grid, hipStream_t s) {
     hipLaunchKernelGGL(copy16_kernel, dim3(grid), dim3(256), 0, s, (uint64_t)(uintptr_t)src,
                        (uint64_t)(uintptr_t)dst, n16);

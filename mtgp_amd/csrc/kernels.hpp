@@ -26,6 +26,12 @@ const void *jump_kernel_fn(int bits, int cls);
 hipError_t launch_jumps(int bits, int cls, const JumpItem *items, int n_items, const DevParam *params,
                         uint32_t *queue, int grid, size_t dyn_lds, hipStream_t s);
 
+// init_kernels.hip: one wave per InitItem (n <= 1727 words).
+hipError_t launch_init(const InitItem *items, int n_items, hipStream_t s);
+
+// Writes `value` to n_words 32-bit words at dst with about `waves`
+// single-wave workgroups, each storing its own contiguous rows.
+hipError_t launch_fill(void *dst, uint64_t n_words, uint32_t value, int waves, hipStream_t s);
 // Copies n16 16-byte units src -> dst (device memory, 16-byte aligned).
 hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, int grid, hipStream_t s);
 

@@ -1506,6 +1506,16 @@ int mtgp_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
     API_END(-1)
 }
 
+int mtgp_device_fill(void *dst, uint32_t value, size_t bytes, void *stream) {
+    API_BEGIN
+    Context &c = context();
+    if (((uintptr_t)dst | bytes) & 3)
+        throw Error("mtgp_device_fill needs 4-byte aligned pointer and size");
+    hip_check(launch_fill(dst, bytes / 4, value, c.cus * 16, (hipStream_t)stream), "fill launch");
+    return 0;
+    API_END(-1)
+}
+
 int mtgp_synchronize(void) {
     API_BEGIN
     context();
@@ -1675,6 +1685,78 @@ int mtgp_genset_init_seed(mtgp_genset_t *s, const uint64_t *seeds) {
             upload_window(s, g, w.data());
         }
     }
+    return 0;
+    API_END(-1)
+}
+
+// Device initialisation (init_kernels.hip): mode 0 seeds, mode 1 keys.
+static void device_init(mtgp_genset *s, const uint64_t *seeds, const uint64_t *keys, const int *offsets,
+                        const int *lengths, hipStream_t st) {
+    const int G = (int)s->bits.size();
+    std::vector<InitItem> items((size_t)G);
+    size_t nkeys = 0;
+    if (keys)
+        for (int g = 0; g < G; g++) {
+            if (offsets[g] < 0 || lengths[g] < 0)
+                throw Error("negative key offset or length");
+            nkeys = std::max(nkeys, (size_t)offsets[g] + (size_t)lengths[g]);
+        }
+    uint64_t *d_keys = nullptr;
+    InitItem *d_items = nullptr;
+    try {
+        hip_check(hipMalloc((void **)&d_items, items.size() * sizeof(InitItem)), "hipMalloc init items");
+        if (nkeys) {
+            hip_check(hipMalloc((void **)&d_keys, nkeys * 8), "hipMalloc keys");
+            hip_check(hipMemcpy(d_keys, keys, nkeys * 8, hipMemcpyHostToDevice), "hipMemcpy keys");
+        }
+        for (int g = 0; g < G; g++) {
+            InitItem &it = items[(size_t)g];
+            it.state = (uint64_t)(uintptr_t)s->state(g, s->cur[(size_t)g]);
+            it.n = s->n[(size_t)g];
+            it.bits = s->bits[(size_t)g];
+            if (it.bits == 32) {
+                const auto &p = s->p32[(size_t)g];
+                it.hidden = p.tbl[4] ^ (p.tbl[8] << 16);
+            } else {
+                const auto &p = s->p64[(size_t)g];
+                it.hidden = p.tbl[4] ^ (p.tbl[8] << 16);
+            }
+            if (keys) {
+                it.mode = 1;
+                it.key_len = lengths[g];
+                it.seed = (uint64_t)(uintptr_t)(d_keys + offsets[g]);
+            } else {
+                it.mode = 0;
+                it.seed = it.bits == 32 ? (uint64_t)(uint32_t)seeds[g] : seeds[g];
+            }
+        }
+        hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(InitItem), hipMemcpyHostToDevice),
+                  "hipMemcpy init items");
+        hip_check(launch_init(d_items, G, st), "init kernel launch");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    } catch (...) {
+        if (d_items)
+            (void)hipFree(d_items);
+        if (d_keys)
+            (void)hipFree(d_keys);
+        throw;
+    }
+    (void)hipFree(d_items);
+    if (d_keys)
+        (void)hipFree(d_keys);
+}
+
+int mtgp_genset_init_seed_device(mtgp_genset_t *s, const uint64_t *seeds, void *stream) {
+    API_BEGIN
+    device_init(s, seeds, nullptr, nullptr, nullptr, (hipStream_t)stream);
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_genset_init_array_device(mtgp_genset_t *s, const uint64_t *keys, const int *offsets, const int *lengths,
+                                  void *stream) {
+    API_BEGIN
+    device_init(s, nullptr, keys, offsets, lengths, (hipStream_t)stream);
     return 0;
     API_END(-1)
 }

@@ -26,3 +26,7 @@ extern "C" int emu_run_jumps(int bits, int cls, const mtgp::JumpItem *items, int
         return -1;
     return (queue[0] == 0 && queue[1] == 0) ? 0 : -2;
 }
+
+extern "C" int emu_run_init(const mtgp::InitItem *items, int n_items) {
+    return mtgp::launch_init(items, n_items, nullptr) == hipSuccess ? 0 : -1;
+}

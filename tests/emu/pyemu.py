@@ -13,6 +13,9 @@ ITEM = np.dtype([("out", "<u8"), ("count", "<u8"), ("state_in", "<u8"), ("state_
                  ("poly", "<u8"), ("poly_words", "<i4"), ("param", "<i4"), ("kind", "<i4"),
                  ("parts", "<i4"), ("part_bytes", "<u8"), ("segs", "<u8"), ("nsegs", "<i4"), ("pad0", "<i4")])
 assert ITEM.itemsize == 80
+INIT_ITEM = np.dtype([("state", "<u8"), ("seed", "<u8"), ("hidden", "<u8"), ("n", "<i4"), ("bits", "<i4"),
+                      ("key_len", "<i4"), ("mode", "<i4"), ("pad", "<u8")])
+assert INIT_ITEM.itemsize == 48
 SEGMENT = np.dtype([("out", "<u8"), ("count", "<u8"), ("kind", "<i4"), ("pad0", "<i4"), ("pad1", "<u8")])
 assert SEGMENT.itemsize == 32
 
@@ -22,6 +25,7 @@ def load():
     lib = C.CDLL(os.path.join(HERE, "libmtgp_emu.so"), mode=os.RTLD_LOCAL)
     lib.emu_run_items.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
     lib.emu_run_jumps.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
+    lib.emu_run_init.argtypes = [C.c_void_p, C.c_int]
     return lib
 
 

@@ -243,3 +243,21 @@ def test_genset_chains_and_gaps(mtgp, oracle, monkeypatch, jump_mode):
             assert_same(buf.to_host(dt, count), expect[k], "request %d" % k)
     for g in range(3):
         assert_same(gs.state(g), state[g][1], "state %d" % g)
+
+
+def test_device_fill_and_copy(mtgp):
+    """The plain fill and copy kernels (counter and ceiling references, and
+    the graph-safe way to clear memory): exact ranges, nothing beyond."""
+    n = (1 << 20) + 12
+    buf = mtgp.DeviceBuffer(4 * n + 64)
+    buf.from_host(np.zeros(n + 16, np.uint32))
+    mtgp.device_fill(buf, 0xDEADBEEF, 4 * (n - 3))
+    mtgp.synchronize()
+    h = buf.to_host(np.uint32, n + 16)
+    assert (h[:n - 3] == 0xDEADBEEF).all() and (h[n - 3:] == 0).all()
+    src = mtgp.DeviceBuffer(4096)
+    src.from_host(np.arange(1024, dtype=np.uint32))
+    mtgp.device_copy(buf, src, 4096)
+    mtgp.synchronize()
+    h = buf.to_host(np.uint32, 1100)
+    assert (h[:1024] == np.arange(1024)).all() and (h[1024:1100] == 0xDEADBEEF).all()

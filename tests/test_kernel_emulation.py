@@ -194,3 +194,32 @@ def test_item_segment_chain(emu, oracle, bits, mexp, layout):
         assert (_bits(got) == _bits(e)).all()
         assert (raw[start + n * eb:] == 0).all() and (raw[:start] == 0).all()
     assert (sout == w).all()
+
+
+@pytest.mark.parametrize("bits,mexp", SETS)
+def test_init_kernel(emu, oracle, bits, mexp):
+    """init_kernels.hip (device init_state / init_by_array), emulated, for
+    key lengths below, at and above the state size, against the oracle."""
+    p = oracle.params(bits, mexp, 3)
+    n = p.n
+    tbl = list(p.tbl)
+    hidden = (tbl[4] ^ (tbl[8] << 16)) & ((1 << bits) - 1)
+    wd = np.uint32 if bits == 32 else np.uint64
+    rng = np.random.default_rng(mexp)
+    cases = [("seed", 12345), ("seed", 2**31 + 7), ("array", 1), ("array", n - 1), ("array", n + 5)]
+    items = np.zeros(len(cases), pyemu.INIT_ITEM)
+    outs, keys, want = [], [], []
+    for k, (mode, v) in enumerate(cases):
+        out = np.zeros(n, wd)
+        outs.append(out)
+        if mode == "seed":
+            items[k] = (out.ctypes.data, v, hidden, n, bits, 0, 0, 0)
+            want.append(oracle.init_state(p, v))
+        else:
+            key = rng.integers(0, 2**63, v, dtype=np.uint64)
+            keys.append(key)
+            items[k] = (out.ctypes.data, key.ctypes.data, hidden, n, bits, v, 1, 0)
+            want.append(oracle.init_by_array(p, key.astype(wd)))
+    assert emu.emu_run_init(items.ctypes.data, len(cases)) == 0
+    for k, (out, w) in enumerate(zip(outs, want)):
+        assert (out == w).all(), cases[k]
