@@ -279,6 +279,8 @@ def main():
     ap.add_argument("--segment", type=int, default=0, help="numbers per segment (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--only-serving", action="store_true",
+                    help="run only the serving comparison and print its JSON (for profiling it alone)")
     ap.add_argument("--serving-rate", type=float, default=20000.0, help="serving requests per second")
     ap.add_argument("--serving-requests", type=int, default=12000)
     ap.add_argument("--no-check", action="store_true")
@@ -287,6 +289,10 @@ def main():
         print("bench.py measures one GPU: every generator of the workload fits one MI355X and the "
               "path has no cross-GPU step; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.only_serving:
+        import mtgp_amd as M
+        print(json.dumps({"serving": run_serving(M, args.serving_requests, args.serving_rate)}), flush=True)
+        return
     r = run_gpu(args)
     gbs = r["out_bytes"] * args.steps / (r["total_ms"] * 1e-3) / 1e9
     # Dominant kernel: the generation launch.  Algorithmic bytes are the
