@@ -883,6 +883,40 @@ __device__ __forceinline__ uint64_t lane_mask64(uint64_t v, int src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// acc[m] ^= XOR of R[q + m] over the set bits q of a wave-uniform mask, one
+// uniform branch per bit.  (Measured against taking the bits in pairs with
+// v_xor3, and against branch-free acc ^ (R & s) with scalar masks: both were
+// 1.8x and 2.6x slower on the batched jump launch.)
+template <int QMAX, int NJ>
+__device__ __forceinline__ void fold_column(uint32_t mask, const uint32_t (&R)[QMAX + NJ], uint32_t (&acc)[NJ]) {
+    if (!mask)
+        return;
+#pragma unroll
+    for (int q = 0; q < QMAX; q++)
+        if (mask >> q & 1u) {
+#pragma unroll
+            for (int m = 0; m < NJ; m++)
+                acc[m] ^= R[q + m];
+        }
+}
+
+// fold_column for the two 32-bit planes of MTGP64 words.
+template <int QMAX, int NJ>
+__device__ __forceinline__ void fold_column2(uint32_t mask, const uint32_t (&RH)[QMAX + NJ],
+                                             const uint32_t (&RL)[QMAX + NJ], uint32_t (&ah)[NJ], uint32_t (&al)[NJ]) {
+    if (!mask)
+        return;
+#pragma unroll
+    for (int q = 0; q < QMAX; q++)
+        if (mask >> q & 1u) {
+#pragma unroll
+            for (int m = 0; m < NJ; m++) {
+                ah[m] ^= RH[q + m];
+                al[m] ^= RL[q + m];
+            }
+        }
+}
+
 template <int NMAX>
 __global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restrict__ items, int n_items,
                                                          const DevParam *__restrict__ params,
@@ -937,18 +971,8 @@ __global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restr
                 for (int r = 0; r < QMAX + NJ; r++)
                     R[r] = col[kWave * r];
 #pragma unroll
-                for (int b = 0; b < B; b++) {
-                    const uint32_t mask = (uint32_t)(mm >> (16 * b)) & 0xffffu;
-                    if (!mask)
-                        continue;
-#pragma unroll
-                    for (int q = 0; q < QMAX; q++)
-                        if (mask >> q & 1u) {
-#pragma unroll
-                            for (int m = 0; m < NJ; m++)
-                                acc[b][m] ^= R[q + m];
-                        }
-                }
+                for (int b = 0; b < B; b++)
+                    fold_column<QMAX, NJ>((uint32_t)(mm >> (16 * b)) & 0xffffu, R, acc[b]);
             }
             relocate32<NJ>(buf, K * kStep, p.n);
         }
@@ -1036,20 +1060,8 @@ __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restr
                     RL[r] = bl[rho + l + kWave * r];
                 }
 #pragma unroll
-                for (int b = 0; b < B; b++) {
-                    const uint32_t mask = (uint32_t)(mm >> (16 * b)) & 0xffffu;
-                    if (!mask)
-                        continue;
-#pragma unroll
-                    for (int q = 0; q < QMAX; q++)
-                        if (mask >> q & 1u) {
-#pragma unroll
-                            for (int m = 0; m < NJ; m++) {
-                                ah[b][m] ^= RH[q + m];
-                                al[b][m] ^= RL[q + m];
-                            }
-                        }
-                }
+                for (int b = 0; b < B; b++)
+                    fold_column2<QMAX, NJ>((uint32_t)(mm >> (16 * b)) & 0xffffu, RH, RL, ah[b], al[b]);
             }
             relocate64<NJ>(bh, bl, K * kStep, p.n);
         }
