@@ -216,7 +216,7 @@ typedef struct mtgp_server_config {
     uint64_t cell;        /* numbers per work item of a large request (power of two; 0: 2^17) */
     int cached_cells;     /* jump polynomials kept per parameter set: x^(k cell), k <= this (0: 512) */
     int streams;          /* device streams (0: 4) */
-    int max_inflight;     /* batches on the device at once (0: 8) */
+    int max_inflight;     /* batches on the device at once (0: 4) */
     int max_batch;        /* requests per batch at most (0: 256) */
     int flags;
 } mtgp_server_config_t;

@@ -649,15 +649,16 @@ bool JumpCache::get(int slot, uint64_t pos, std::vector<uint32_t> &words, bool c
 
 namespace {
 // Grow-only device / pinned buffers (a plan's buffers are reused by later
-// builds; freeing synchronises, so it only happens when one must grow).
+// builds; freeing synchronises the device, so they grow rarely: at least
+// double, at least 1 MiB).
 void grow_device(char *&p, size_t &cap, size_t need, const char *what) {
     if (need <= cap)
         return;
     if (p)
         hip_check(hipFree(p), "hipFree");
     p = nullptr;
+    const size_t want = std::max({need, 2 * cap, (size_t)1 << 20});
     cap = 0;
-    const size_t want = std::max(need, (size_t)1 << 20);
     hip_check(hipMalloc((void **)&p, want), what);
     cap = want;
 }
@@ -667,8 +668,8 @@ void grow_pinned(char *&p, size_t &cap, size_t need) {
     if (p)
         hip_check(hipHostFree(p), "hipHostFree");
     p = nullptr;
+    const size_t want = std::max({need, 2 * cap, (size_t)1 << 20});
     cap = 0;
-    const size_t want = std::max(need, (size_t)1 << 20);
     hip_check(hipHostMalloc((void **)&p, want, hipHostMallocDefault), "hipHostMalloc staging");
     cap = want;
 }
@@ -911,7 +912,8 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     int parts = 1;
     if (!groups.empty()) {
         const int target = c.cus * 16;
-        parts = std::max(1, std::min<int>((target + (int)groups.size() - 1) / (int)groups.size(), 8));
+        parts = std::max(1, std::min<int>((target + (int)groups.size() - 1) / (int)groups.size(),
+                                          std::max(1, opt.max_parts)));
     }
     std::vector<int> jgroup;
     jitems.clear();
@@ -1114,6 +1116,16 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         throw Error("too many launches in one plan");
     segment_used = S;
     plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void Plan::reserve(size_t work_bytes, size_t jstates_bytes) {
+    grow_device(d_work, work_cap, work_bytes, "hipMalloc plan workspace");
+    grow_pinned(h_stage, stage_cap, work_bytes);
+    grow_device(d_jstates, jstates_cap, jstates_bytes, "hipMalloc state arena");
+    if (!d_queue) {
+        hip_check(hipMalloc((void **)&d_queue, 64 * sizeof(uint32_t)), "hipMalloc queue");
+        hip_check(hipMemset(d_queue, 0, 64 * sizeof(uint32_t)), "hipMemset queue");
+    }
 }
 
 void Plan::launch(hipStream_t s) {

@@ -144,6 +144,7 @@ struct PlanOptions {
     hipStream_t stream = nullptr;  // upload stream
     bool sync = true;              // wait for the upload before returning
     int jump_mode = -1;            // 0 fused, 1 batched, -1 the plan's default
+    int max_parts = 8;             // waves a batched jump is split over, at most
 };
 
 // A list of device Items plus the launches that run them.
@@ -194,6 +195,9 @@ struct Plan {
         o.segment = segment;
         build(c, gens, reqs, o);
     }
+    // Sizes the grow-only buffers up front (a plan that is rebuilt often
+    // then never frees, which would synchronise the device).
+    void reserve(size_t work_bytes, size_t jstates_bytes);
     void launch(hipStream_t s);
     void launch_part(int part, hipStream_t s);  // 0 jumps, 1 generation
     // Overrides the step layout and/or the resident workgroups per CU of

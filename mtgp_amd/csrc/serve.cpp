@@ -263,6 +263,9 @@ struct mtgp_server {
         o.stream = streams[(size_t)s];
         o.sync = false;
         o.jump_mode = (cfg.flags & MTGP_SERVER_FUSED_JUMPS) ? 0 : 1;
+        // Batches carry few jumps: split each down to one chunk per wave so
+        // a jump takes about one chunk's fold instead of several.
+        o.max_parts = 64;
         const auto tp = Clock::now();
         B.plan.build(*ctx, set->refs(), rq, o);
         stats.plan_us += std::chrono::duration<double, std::micro>(Clock::now() - tp).count();
@@ -488,7 +491,8 @@ mtgp_server_t *mtgp_server_create(mtgp_genset_t *set, const mtgp_server_config_t
     if (!c.streams)
         c.streams = 4;
     if (!c.max_inflight)
-        c.max_inflight = 8;
+        c.max_inflight = 4;  // one per stream: more queue behind each other on the
+                             // device's hardware queues and lengthen the tail
     if (!c.max_batch)
         c.max_batch = 256;
     if (c.streams < 1 || c.streams > 8 || c.max_inflight < 1 || c.max_inflight > 64 || c.max_batch < 1)
@@ -508,6 +512,8 @@ mtgp_server_t *mtgp_server_create(mtgp_genset_t *set, const mtgp_server_config_t
     for (int k = 0; k < c.max_inflight; k++) {
         auto b = std::make_unique<Batch>();
         hip_check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
+        b->plan.ctx = s->ctx;
+        b->plan.reserve((size_t)8 << 20, (size_t)64 << 20);
         s->pool.push_back(std::move(b));
     }
     s->last_batch.assign(set->bits.size(), -1);
