@@ -45,11 +45,10 @@ enum {
 /* ------------------------------------------- dynamic-creator parameters
  * The 200-set MEXP-11213 tables of the CUDA sample
  * (cuda-sample/mtgp32dc-param-11213.c, mtgp64dc-param-11213.c), usable
- * with the init family, the inline genrand family and mtgp*_fast_jump.
- * MTGP32-DC sets also run on the GPU (mtgp32_fill, generator sets via
- * mtgp_genset_create_params).  MTGP64-DC sets have N - pos < 256 (N = 176)
- * and are host-only: the device step computes 256 words at a time, so the
- * GPU entry points reject them with an error. */
+ * with the init family, the inline genrand family, mtgp*_fast_jump, the
+ * bulk fills and generator sets (mtgp_genset_create_params).  MTGP64-DC
+ * sets have N = 176 and N - pos < 256; they run on kernels that step 128
+ * words at a time. */
 extern mtgp32_params_fast_t mtgp32dc_params_fast_11213[200];
 extern mtgp64_params_fast_t mtgp64dc_params_fast_11213[200];
 

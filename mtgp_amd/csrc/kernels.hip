@@ -551,33 +551,33 @@ __device__ __forceinline__ void step64(uint32_t *bh, uint32_t *bl, const uint32_
 
 // Lane-strided variant of step64 (see step32s); outputs are 8-byte stores,
 // 512 contiguous bytes per instruction.
-template <int KIND, bool OUT, bool FULL = false>
+template <int KIND, bool OUT, bool FULL = false, int WPL = 4>
 __device__ __forceinline__ void step64s(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b,
                                         gu32 *dst, uint32_t m) {
     const int l = threadIdx.x;
     const int q = b + l;
     const int yq = q + p.pos - 1;
-    uint32_t rh[4], rl[4], tl[4];
+    uint32_t rh[WPL], rl[WPL], tl[WPL];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < WPL; i++) {
         const int x = q + kWave * i, y = yq + kWave * i;
         tl[i] = bl[y];
         rec64(p, tab, bh[x], bl[x], bh[x + 1], bl[x + 1], bh[y + 1], bl[y + 1], rh[i], rl[i]);
     }
     const int w = q + p.n;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < WPL; i++) {
         bh[w + kWave * i] = rh[i];
         bl[w + kWave * i] = rl[i];
     }
     if (OUT) {
         gu2 *d2 = reinterpret_cast<gu2 *>(dst);
-        uint32_t oh[4], ol[4];
+        uint32_t oh[WPL], ol[WPL];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < WPL; i++)
             out64<KIND>(tab, rh[i], rl[i], tl[i], oh[i], ol[i]);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < WPL; i++) {
             const uint32_t e = (uint32_t)(l + kWave * i);
             if (FULL || e < m)
                 d2[e] = v2u{ol[i], oh[i]};
@@ -684,44 +684,50 @@ __device__ __forceinline__ uint32_t head64(uint64_t out, uint64_t count) {
     return (uint64_t)h > count ? (uint32_t)count : h;
 }
 
-template <int NJ, int NBUF, int LAYOUT>
+// WPL (words per lane and step): 4 = 256-word steps; 2 = 128-word steps for
+// states with N - pos < 256 (the MTGP64 dynamic-creator sets, N = 176),
+// lane-strided layout only.
+template <int NJ, int NBUF, int LAYOUT, int WPL = 4>
 __device__ __forceinline__ int place64(uint32_t *bh, uint32_t *bl, const P64 &p, int b, uint32_t h) {
     const int to = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
     const bool misaligned = LAYOUT == 0 && (((uint32_t)b + h) & 3u) != 0;
-    if (misaligned || b + p.n + kStep > NBUF) {
+    if (misaligned || b + p.n + kWave * WPL > NBUF) {
         relocate64<NJ>(bh, bl, b, p.n, to);
         return to;
     }
     return b;
 }
 
-template <int NJ, int NBUF>
+template <int NJ, int NBUF, int WPL = 4>
 __device__ int advance64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b, uint64_t count) {
-    for (uint64_t s = count / kStep; s > 0; s--) {
-        if (b + p.n + kStep > NBUF) {
+    constexpr int W = kWave * WPL;
+    for (uint64_t s = count / W; s > 0; s--) {
+        if (b + p.n + W > NBUF) {
             relocate64<NJ>(bh, bl, b, p.n);
             b = 0;
         }
-        step64s<0, false>(bh, bl, tab, p, b, nullptr, 0);
+        step64s<0, false, false, WPL>(bh, bl, tab, p, b, nullptr, 0);
         wave_sync();
-        b += kStep;
+        b += W;
     }
-    const uint32_t m = (uint32_t)(count % kStep);
+    const uint32_t m = (uint32_t)(count % W);
     if (m) {
-        if (b + p.n + kStep > NBUF) {
+        if (b + p.n + W > NBUF) {
             relocate64<NJ>(bh, bl, b, p.n);
             b = 0;
         }
-        step64s<0, false>(bh, bl, tab, p, b, nullptr, 0);
+        step64s<0, false, false, WPL>(bh, bl, tab, p, b, nullptr, 0);
         wave_sync();
         b += (int)m;
     }
     return b;
 }
 
-template <int KIND, int NJ, int NBUF, int LAYOUT>
+template <int KIND, int NJ, int NBUF, int LAYOUT, int WPL = 4>
 __device__ int generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b, uint32_t h,
                           gu32 *__restrict__ out, uint64_t count) {
+    static_assert(LAYOUT == 1 || WPL == 4, "128-word steps are lane-strided");
+    constexpr int W = kWave * WPL;
     const int l = threadIdx.x;
     gu32 *dst = out;
     uint64_t rest = count;
@@ -737,67 +743,67 @@ __device__ int generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const
                 reinterpret_cast<gu2 *>(out)[l] = v2u{ol, oh};
             }
         } else {
-            step64s<KIND, true>(bh, bl, tab, p, b, dst, h);
+            step64s<KIND, true, false, WPL>(bh, bl, tab, p, b, dst, h);
         }
         wave_sync();
         b += (int)h;
         dst += 2 * h;
         rest -= (uint64_t)h;
     }
-    for (uint64_t s = rest / kStep; s > 0; s--) {
-        if (b + p.n + kStep > NBUF) {
+    for (uint64_t s = rest / W; s > 0; s--) {
+        if (b + p.n + W > NBUF) {
             relocate64<NJ>(bh, bl, b, p.n);
             b = 0;
         }
         if (LAYOUT == 0)
             step64<KIND, true, true>(bh, bl, tab, p, b, dst, kStep);
         else
-            step64s<KIND, true, true>(bh, bl, tab, p, b, dst, kStep);
+            step64s<KIND, true, true, WPL>(bh, bl, tab, p, b, dst, W);
         wave_sync();
-        b += kStep;
-        dst += 2 * kStep;
+        b += W;
+        dst += 2 * W;
     }
-    const uint32_t m = (uint32_t)(rest % kStep);
+    const uint32_t m = (uint32_t)(rest % W);
     if (m) {
-        if (b + p.n + kStep > NBUF) {
+        if (b + p.n + W > NBUF) {
             relocate64<NJ>(bh, bl, b, p.n);
             b = 0;
         }
         if (LAYOUT == 0)
             step64<KIND, true>(bh, bl, tab, p, b, dst, m);
         else
-            step64s<KIND, true>(bh, bl, tab, p, b, dst, m);
+            step64s<KIND, true, false, WPL>(bh, bl, tab, p, b, dst, m);
         wave_sync();
         b += (int)m;
     }
     return b;
 }
 
-template <int NJ, int NBUF, int LAYOUT>
+template <int NJ, int NBUF, int LAYOUT, int WPL = 4>
 __device__ int segment64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b, uint64_t out,
                          uint64_t count, int kind) {
     if (count == 0)
         return b;
     if (out == 0)
-        return advance64<NJ, NBUF>(bh, bl, tab, p, b, count);
+        return advance64<NJ, NBUF, WPL>(bh, bl, tab, p, b, count);
     const uint32_t h = head64<LAYOUT>(out, count);
-    b = place64<NJ, NBUF, LAYOUT>(bh, bl, p, b, h);
+    b = place64<NJ, NBUF, LAYOUT, WPL>(bh, bl, p, b, h);
     gu32 *dst = reinterpret_cast<gu32 *>(out);
     switch (kind) {
-    case 0: return generate64<0, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
-    case 1: return generate64<1, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
-    case 2: return generate64<2, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
-    case 3: return generate64<3, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
-    default: return generate64<4, NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, h, dst, count);
+    case 0: return generate64<0, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
+    case 1: return generate64<1, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
+    case 2: return generate64<2, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
+    case 3: return generate64<3, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
+    default: return generate64<4, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
     }
 }
 
-template <int NMAX, int K, int LAYOUT>
+template <int NMAX, int K, int LAYOUT, int WPL = 4>
 __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict__ items, int n_items,
                                                           const DevParam *__restrict__ params,
                                                           uint32_t *__restrict__ queue) {
     constexpr int NJ = (NMAX + kWave - 1) / kWave;
-    constexpr int NBUF = K * kStep + kWave * NJ + kWave;
+    constexpr int NBUF = K * kWave * WPL + kWave * NJ + kWave;
     __shared__ __attribute__((aligned(16))) uint32_t bh[NBUF];
     __shared__ __attribute__((aligned(16))) uint32_t bl[NBUF];
     __shared__ uint32_t tab[48];
@@ -818,7 +824,9 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         gu64 *state_out = reinterpret_cast<gu64 *>(I.state_out);
         const gu64 *state_in = reinterpret_cast<const gu64 *>(I.state_in);
         wave_sync();
-        if (I.poly) {
+        // (128-word steps: the host gives such items batched start states, so
+        // the fused jump, which steps 256 words at a time, is compiled out.)
+        if (WPL == 4 && I.poly) {
             uint32_t ah[NJ], al[NJ];
             jump64<NJ, K, NBUF>(bh, bl, tab, p, state_in, reinterpret_cast<const gu32 *>(I.poly),
                                 I.poly_words, ah, al);
@@ -844,9 +852,10 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         if (I.nsegs > 0) {
             const GMEM Segment *segs = reinterpret_cast<const GMEM Segment *>(I.segs);
             for (int si = 0; si < I.nsegs; si++)
-                b = segment64<NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, segs[si].out, segs[si].count, segs[si].kind);
+                b = segment64<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, segs[si].out, segs[si].count,
+                                                     segs[si].kind);
         } else {
-            b = segment64<NJ, NBUF, LAYOUT>(bh, bl, tab, p, b, I.out, I.count, I.kind);
+            b = segment64<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, I.out, I.count, I.kind);
         }
         if (state_out)
             for (int j = l; j < p.n; j += kWave)
@@ -1001,15 +1010,15 @@ __global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restr
     }
 }
 
-template <int NMAX>
+template <int NMAX, int WPL = 4>
 __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restrict__ items, int n_items,
                                                          const DevParam *__restrict__ params,
                                                          uint32_t *__restrict__ queue) {
-    constexpr int K = kJumpSteps;
+    constexpr int K = kJumpChunk / (kWave * WPL);  // steps per chunk
     constexpr int NJ = (NMAX + kWave - 1) / kWave;
-    constexpr int QMAX = 4 * K;
+    constexpr int QMAX = kJumpChunk / kWave;
     constexpr int B = kJumpBatch;
-    constexpr int NBUF = K * kStep + kWave * NJ + kWave;
+    constexpr int NBUF = kJumpChunk + kWave * NJ + kWave;
     __shared__ __attribute__((aligned(16))) uint32_t bh[NBUF];
     __shared__ __attribute__((aligned(16))) uint32_t bl[NBUF];
     __shared__ uint32_t tab[48];
@@ -1044,7 +1053,7 @@ __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restr
                 ah[b][m] = al[b][m] = 0;
         for (int c = 0; c < I.c1; c++) {
             for (int s = 0; s < K; s++) {
-                step64s<0, false>(bh, bl, tab, p, s * kStep, nullptr, 0);
+                step64s<0, false, false, WPL>(bh, bl, tab, p, s * kWave * WPL, nullptr, 0);
                 wave_sync();
             }
             // Lane rho holds residue rho's masks; readlane hands them out.
@@ -1063,7 +1072,7 @@ __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restr
                 for (int b = 0; b < B; b++)
                     fold_column2<QMAX, NJ>((uint32_t)(mm >> (16 * b)) & 0xffffu, RH, RL, ah[b], al[b]);
             }
-            relocate64<NJ>(bh, bl, K * kStep, p.n);
+            relocate64<NJ>(bh, bl, kJumpChunk, p.n);
         }
 #pragma unroll
         for (int b = 0; b < B; b++) {
@@ -1096,15 +1105,18 @@ __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restr
 #endif
 constexpr int kK = MTGP_RELOC_STEPS;
 
-template <int B, int NMAX, int LAYOUT>
+// Largest state of the 64-bit narrow class (128-word steps).
+constexpr int kNarrowN = 176;
+
+template <int B, int NMAX, int LAYOUT, int WPL = 4>
 struct KernelOf;
 template <int NMAX, int LAYOUT>
-struct KernelOf<32, NMAX, LAYOUT> {
+struct KernelOf<32, NMAX, LAYOUT, 4> {
     static constexpr auto fn = mtgp32_items_kernel<NMAX, kK, LAYOUT>;
 };
-template <int NMAX, int LAYOUT>
-struct KernelOf<64, NMAX, LAYOUT> {
-    static constexpr auto fn = mtgp64_items_kernel<NMAX, kK, LAYOUT>;
+template <int NMAX, int LAYOUT, int WPL>
+struct KernelOf<64, NMAX, LAYOUT, WPL> {
+    static constexpr auto fn = mtgp64_items_kernel<NMAX, kK, LAYOUT, WPL>;
 };
 
 template <class KO>
@@ -1115,8 +1127,11 @@ hipError_t launch_one(const Item *items, int n_items, const DevParam *params, ui
 }
 
 // Calls f.template operator()<KernelOf<...>>() for (bits, cls, layout).
+// Class 3 (64-bit only) is the narrow class: 128-word steps, layout 1.
 template <class F>
 auto with_kernel(int bits, int cls, int layout, F &&f) {
+    if (bits == 64 && cls == 3)
+        return f.template operator()<KernelOf<64, kNarrowN, 1, 2>>();
 #define MTGP_CASE(B, N, L) \
     if (bits == B && N_OF(B, cls) == N && layout == L) return f.template operator()<KernelOf<B, N, L>>();
 #define N_OF(B, c) ((B) == 32 ? ((c) == 0 ? 351 : (c) == 1 ? 726 : 1391) : ((c) == 0 ? 363 : (c) == 1 ? 696 : 1727))
@@ -1137,6 +1152,14 @@ int kernel_class(int bits, int n) {
     return n <= 363 ? 0 : n <= 696 ? 1 : n <= 1727 ? 2 : -1;
 }
 
+int kernel_class(int bits, int n, int pos) {
+    if (n - pos >= 256)
+        return kernel_class(bits, n);
+    if (bits == 64 && n <= kNarrowN && n - pos >= 128)
+        return 3;
+    return -1;
+}
+
 const void *kernel_fn(int bits, int cls, int layout) {
     return with_kernel(bits, cls, layout, []<class KO>() { return reinterpret_cast<const void *>(KO::fn); });
 }
@@ -1152,7 +1175,10 @@ template <class F>
 auto with_jump_kernel(int bits, int cls, F &&f) {
     if (bits == 32)
         return cls == 0 ? f(mtgp32_jump_kernel<351>) : cls == 1 ? f(mtgp32_jump_kernel<726>) : f(mtgp32_jump_kernel<1391>);
-    return cls == 0 ? f(mtgp64_jump_kernel<363>) : cls == 1 ? f(mtgp64_jump_kernel<696>) : f(mtgp64_jump_kernel<1727>);
+    return cls == 0   ? f(mtgp64_jump_kernel<363>)
+           : cls == 1 ? f(mtgp64_jump_kernel<696>)
+           : cls == 2 ? f(mtgp64_jump_kernel<1727>)
+                      : f(mtgp64_jump_kernel<kNarrowN, 2>);
 }
 }  // namespace
 

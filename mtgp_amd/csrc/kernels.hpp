@@ -5,8 +5,14 @@
 
 namespace mtgp {
 
-// Kernel class for a state of n words (-1 if none fits).
+// Kernel class for a state of n words (-1 if none fits), assuming
+// N - pos >= 256 (every table set).
 int kernel_class(int bits, int n);
+// The same for any set: 64-bit states with N <= 176 and 128 <= N - pos <
+// 256 (the MTGP64 dynamic-creator sets) get class 3, whose kernels step 128
+// words at a time (lane-strided layout only, batched start-state jumps).
+int kernel_class(int bits, int n, int pos);
+constexpr int kClasses = 4;  // classes 0..2 of both word sizes, 3 for 64-bit
 // Step layouts: 0 = each lane computes 4 consecutive words and stores them
 // with one 128-bit store; 1 = lane l computes words l + 64i (conflict-free
 // LDS, 256-byte coalesced dword stores).

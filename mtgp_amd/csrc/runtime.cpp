@@ -82,15 +82,19 @@ Context &context() {
               "hipMemcpy params");
     c->n_params = kNumSets;
     for (int b = 0; b < 2; b++)
-        for (int cls = 0; cls < 3; cls++)
+        for (int cls = 0; cls < kClasses; cls++)
             for (int lay = 0; lay < kLayouts; lay++) {
+                if (b == 0 && cls == 3)
+                    continue;
                 int nb = 0;
                 hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_fn(b ? 64 : 32, cls, lay), 64, 0),
                           "hipOccupancyMaxActiveBlocksPerMultiprocessor");
                 c->blocks_per_cu[b][cls][lay] = nb > 0 ? nb : 1;
             }
     for (int b = 0; b < 2; b++)
-        for (int cls = 0; cls < 3; cls++) {
+        for (int cls = 0; cls < kClasses; cls++) {
+            if (b == 0 && cls == 3)
+                continue;
             int nb = 0;
             hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, jump_kernel_fn(b ? 64 : 32, cls), 64, 0),
                       "hipOccupancyMaxActiveBlocksPerMultiprocessor");
@@ -102,7 +106,7 @@ Context &context() {
             c->default_bpc = v;
     }
     for (int b = 0; b < 2; b++)
-        for (int cls = 0; cls < 3; cls++)
+        for (int cls = 0; cls < kClasses; cls++)
             c->bpc[b][cls] = c->default_bpc;
     if (const char *e = getenv("MTGP_SEGMENT_SPLIT")) {
         const double v = atof(e);
@@ -118,7 +122,7 @@ Context &context() {
         const int lay = atoi(e);
         if (lay >= 0 && lay < kLayouts)
             for (int b = 0; b < 2; b++)
-                for (int cls = 0; cls < 3; cls++)
+                for (int cls = 0; cls < 3; cls++)  // (class 3 is lane-strided only)
                     c->layout[b][cls] = lay;
     }
     Context &ref = *c;
@@ -170,12 +174,13 @@ size_t Context::lds_reserve(const void *fn, int per_cu) {
 // Parameter slot of an arbitrary (host) parameter record: a table set when
 // it equals one, else a custom slot uploaded on first use.
 int Context::slot_of(const DevParam &rec, const unsigned char *sha1) {
-    // The device step computes 256 words at once, which needs N - pos >= 256
-    // (every table set has it; the MTGP64-DC sets, N = 176, do not).
-    if (rec.n - rec.pos < 256)
-        throw Error("parameter set with N - pos = " + std::to_string(rec.n - rec.pos) +
-                    " < 256 cannot run on the device (MTGP" + std::to_string(rec.bits) + "-" +
-                    std::to_string(rec.mexp) + ", pos " + std::to_string(rec.pos) + ")");
+    // A device step computes 256 words at once, or 128 for the narrow class
+    // (MTGP64 states of N <= 176, like the dynamic-creator sets), which needs
+    // N - pos >= 256 (128).
+    if (kernel_class(rec.bits, rec.n, rec.pos) < 0)
+        throw Error("parameter set with N = " + std::to_string(rec.n) + ", N - pos = " +
+                    std::to_string(rec.n - rec.pos) + " fits no device kernel (MTGP" + std::to_string(rec.bits) +
+                    "-" + std::to_string(rec.mexp) + ")");
     for (int k = 0; k < kNumSets; k++)
         if (memcmp(&host_param(k), &rec, sizeof(DevParam)) == 0)
             return k;
@@ -191,6 +196,11 @@ int Context::slot_of(const DevParam &rec, const unsigned char *sha1) {
     memcpy(cst.sha1, sha1, 20);
     customs.push_back(cst);
     return n_params++;
+}
+
+int Context::class_of(int slot) const {
+    const DevParam &d = param(slot);
+    return kernel_class(d.bits, d.n, d.pos);
 }
 
 const DevParam &Context::param(int slot) const {
@@ -713,7 +723,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     // Resident workgroups of the default configuration of the largest class.
     int slots = 0;
     for (const auto &g : gens) {
-        const int b = g.bits == 64, cls = kernel_class(g.bits, g.n);
+        const int b = g.bits == 64, cls = c.class_of(g.slot);
         slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][c.layout[b][cls]]));
     }
     uint64_t S1 = 0, S2 = 0;
@@ -817,6 +827,12 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         jump_mode = opt.jump_mode ? 1 : 0;
     else if (const char *e = getenv("MTGP_JUMP_MODE"))
         jump_mode = atoi(e) ? 1 : 0;
+    // Launch group of a generator: 0..2 = MTGP32 classes, 3..6 = MTGP64
+    // classes 0..3.  Narrow-class items need batched start states.
+    auto group_of = [&](const GenRef &g) { return (g.bits == 64 ? 3 : 0) + c.class_of(g.slot); };
+    for (const auto &g : gens)
+        if (g.bits == 64 && c.class_of(g.slot) == 3)
+            jump_mode = 1;
     const bool masks = jump_mode == 1;
     std::vector<std::vector<std::vector<uint32_t>>> polys((size_t)G);
     std::vector<std::vector<std::vector<uint16_t>>> m16((size_t)G);
@@ -939,7 +955,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             ji.param = ref.slot;
             ji.state_bytes = (int32_t)sbytes;
             jitems.push_back(ji);
-            jgroup.push_back((ref.bits == 64 ? 3 : 0) + kernel_class(ref.bits, ref.n));
+            jgroup.push_back(group_of(ref));
         }
         arena_bytes += align256((size_t)P * nb * sbytes);
     }
@@ -1005,7 +1021,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             list.push_back(it);
             arena_slot.push_back(slot);
             cost.push_back(n + (it.poly != UINT64_MAX ? 200000u : 0u));
-            group.push_back((gr.bits == 64 ? 3 : 0) + kernel_class(gr.bits, gr.n));
+            group.push_back(group_of(gr));
             if (it.state_out)
                 commits.push_back(g);
         }
@@ -1089,7 +1105,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
                 j++;
             Launch L;
             L.bits = gi >= 3 ? 64 : 32;
-            L.cls = gi % 3;
+            L.cls = gi >= 3 ? gi - 3 : gi;
             L.first = (int)i;
             L.count = (int)(j - i);
             if (jump) {
@@ -1325,8 +1341,7 @@ static void device_jump(Context &c, int slot, int bits, int mexp, const void *in
         hip_check(hipMemcpy(d_item, &ji, sizeof(JumpItem), hipMemcpyHostToDevice), "hipMemcpy");
         hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
         hip_check(hipMemset(d_q, 0, 8), "hipMemset");
-        hip_check(launch_jumps(bits, kernel_class(bits, state_words(bits, mexp)), d_item, 1, c.d_params, d_q, 1, 0, 0),
-                  "jump kernel launch");
+        hip_check(launch_jumps(bits, c.class_of(slot), d_item, 1, c.d_params, d_q, 1, 0, 0), "jump kernel launch");
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (...) {
         cleanup();
@@ -1619,8 +1634,8 @@ static mtgp_genset_t *genset_of(int count, const int *bits, const std::vector<co
         } else {
             throw Error("word size must be 32 or 64");
         }
-        if (kernel_class(bits[g], state_words(bits[g], mexp)) < 0)
-            throw Error("state of MEXP " + std::to_string(mexp) + " is larger than any kernel class");
+        if (c.class_of(slot) < 0)
+            throw Error("parameter set fits no device kernel (MEXP " + std::to_string(mexp) + ")");
         s->bits.push_back(bits[g]);
         s->mexp.push_back(mexp);
         s->slot.push_back(slot);
@@ -2002,7 +2017,7 @@ int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) 
         hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(JumpItem), hipMemcpyHostToDevice), "hipMemcpy");
         hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
         hip_check(hipMemset(d_q, 0, 8), "hipMemset");
-        const int bits = s->bits[0], cls = kernel_class(bits, s->n[0]);
+        const int bits = s->bits[0], cls = c.class_of(s->slot[0]);
         const int n_items = (int)items.size();
         const int grid = std::min(n_items, c.cus * c.jump_blocks_per_cu[bits == 64][cls]);
         hip_check(launch_jumps(bits, cls, d_items, n_items, c.d_params, d_q, grid, 0, 0), "jump kernel launch");

@@ -33,16 +33,17 @@ struct Context {
     std::string arch;
     DevParam *d_params = nullptr;
     int n_params = 0, param_capacity = 0;
-    // Occupancy limit [bits==64][class][layout] and the default layout.
-    int blocks_per_cu[2][3][2] = {};
-    int jump_blocks_per_cu[2][3] = {};
-    int layout[2][3] = {{1, 1, 1}, {1, 1, 1}};
+    // Occupancy limit [bits==64][class][layout] and the default layout
+    // (class 3 exists for 64-bit only; kernels.hpp).
+    int blocks_per_cu[2][4][2] = {};
+    int jump_blocks_per_cu[2][4] = {};
+    int layout[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
     // Resident workgroups per CU used by default (measured best for the
     // generation kernels: 16 of the 32 the occupancy limit allows).
     int default_bpc = 16;
     // Resident workgroups per CU for generation launches per [bits==64][class]
     // (default_bpc until the autotuner measures better).
-    int bpc[2][3] = {{16, 16, 16}, {16, 16, 16}};
+    int bpc[2][4] = {{16, 16, 16, 16}, {16, 16, 16, 16}};
     // Automatic segmentation cuts the work of a plan into about this many
     // segments per resident workgroup slot: more rounds balance the tail
     // better, each extra segment costs one jump-ahead (1.5 measured best for
@@ -68,6 +69,8 @@ struct Context {
     size_t lds_reserve(const void *fn, int per_cu);
     int slot_of(const DevParam &rec, const unsigned char *sha1);
     const DevParam &param(int slot) const;
+    // Kernel class of a parameter slot (from its N and pos).
+    int class_of(int slot) const;
     void sha1_of(int slot, unsigned char out[20]) const;
 };
 

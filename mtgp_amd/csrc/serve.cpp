@@ -518,7 +518,7 @@ mtgp_server_t *mtgp_server_create(mtgp_genset_t *set, const mtgp_server_config_t
     }
     s->last_batch.assign(set->bits.size(), -1);
     for (size_t g = 0; g < set->bits.size(); g++)
-        s->gen_group.push_back((set->bits[g] == 64 ? 3 : 0) + mtgp::kernel_class(set->bits[g], set->n[g]));
+        s->gen_group.push_back((set->bits[g] == 64 ? 3 : 0) + s->ctx->class_of(set->slot[g]));
     // Finish whatever the set's own calls left in flight before the
     // scheduler starts reading its states.
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");

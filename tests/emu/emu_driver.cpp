@@ -30,3 +30,21 @@ extern "C" int emu_run_jumps(int bits, int cls, const mtgp::JumpItem *items, int
 extern "C" int emu_run_init(const mtgp::InitItem *items, int n_items) {
     return mtgp::launch_init(items, n_items, nullptr) == hipSuccess ? 0 : -1;
 }
+
+// The same two with a caller-supplied parameter table (sets outside the
+// library's tables, e.g. dynamic-creator sets).
+extern "C" int emu_run_items_p(int bits, int cls, int layout, const mtgp::Item *items, int n_items, int grid,
+                               const mtgp::DevParam *params) {
+    uint32_t queue[2] = {0, 0};
+    if (mtgp::launch_items(bits, cls, layout, items, n_items, params, queue, grid, 0, nullptr) != hipSuccess)
+        return -1;
+    return (queue[0] == 0 && queue[1] == 0) ? 0 : -2;
+}
+
+extern "C" int emu_run_jumps_p(int bits, int cls, const mtgp::JumpItem *items, int n_items, int grid,
+                               const mtgp::DevParam *params) {
+    uint32_t queue[2] = {0, 0};
+    if (mtgp::launch_jumps(bits, cls, items, n_items, params, queue, grid, 0, nullptr) != hipSuccess)
+        return -1;
+    return (queue[0] == 0 && queue[1] == 0) ? 0 : -2;
+}

@@ -26,7 +26,30 @@ def load():
     lib.emu_run_items.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
     lib.emu_run_jumps.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
     lib.emu_run_init.argtypes = [C.c_void_p, C.c_int]
+    lib.emu_run_items_p.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    lib.emu_run_jumps_p.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     return lib
+
+
+DEVPARAM = np.dtype([("bits", "<i4"), ("mexp", "<i4"), ("n", "<i4"), ("pos", "<i4"), ("sh1", "<i4"), ("sh2", "<i4"),
+                     ("mask_lo", "<u4"), ("mask_hi", "<u4"), ("tbl", "<u4", (16,)), ("tmp", "<u4", (16,)),
+                     ("flt", "<u4", (16,))])
+assert DEVPARAM.itemsize == 224
+
+
+def devparam(bits, rec):
+    """The kernels' parameter record from a 73-value reference record
+    (64-bit tables keep the high halves, whose low halves are zero)."""
+    rec = [int(v) for v in rec]
+    d = np.zeros(1, DEVPARAM)
+    d["bits"], d["mexp"], d["n"] = bits, rec[0], rec[0] // bits + 1
+    d["pos"], d["sh1"], d["sh2"] = rec[1], rec[2], rec[3]
+    shift = 32 if bits == 64 else 0
+    d["tbl"][0] = [v >> shift for v in rec[4:20]]
+    d["tmp"][0] = [v >> shift for v in rec[20:36]]
+    d["flt"][0] = [v >> shift for v in rec[36:52]]
+    d["mask_lo"], d["mask_hi"] = rec[52] & 0xFFFFFFFF, rec[52] >> 32
+    return d
 
 
 JUMP_ITEM = np.dtype([("state_in", "<u8"), ("masks", "<u8"), ("states_out", "<u8"), ("nb", "<i4"),

@@ -213,3 +213,28 @@ def test_dropin_headers_reproduce_reference_programs(tmp_path):
                 sha = bytes(gp["records"]["%d/%d/%d" % (bits, mexp, idx)][53:73]).hex()
                 assert line == "mtgp%d:%d:%s\n" % (bits, mexp, sha)
     assert pos == len(data)
+
+
+def test_gpu_entry_points_fail_loudly_without_gpu(mtgp):
+    """No CPU fallback: without a visible GPU every entry point that computes
+    numbers in bulk (and the device jump) returns an error and computes
+    nothing; the host init family still works."""
+    try:
+        mtgp.device_info()
+    except mtgp.MTGPError:
+        pass
+    else:
+        pytest.skip("a GPU is visible")
+    g = mtgp.Generator32(11213, 0, seed=1)
+    w0 = g.window().copy()
+    out = np.zeros(1000, np.uint32)
+    with pytest.raises(mtgp.MTGPError):
+        g.fill(0, 1000, out=out)
+    assert (out == 0).all() and (g.window() == w0).all()
+    with pytest.raises(mtgp.MTGPError):
+        g.fast_jump(mtgp.jump_hex(32, 11213, 0, 12345))
+    assert (g.window() == w0).all()
+    with pytest.raises(mtgp.MTGPError):
+        mtgp.GenSet([(32, 11213, 0)])
+    with pytest.raises(mtgp.MTGPError):
+        mtgp.DeviceBuffer(1024)

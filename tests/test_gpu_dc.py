@@ -1,5 +1,6 @@
-"""Dynamic-creator MTGP32 sets on the GPU, against the reference's outputs
-(tests/golden/dc_outputs.npz); MTGP64-DC sets are rejected on the device."""
+"""Dynamic-creator sets on the GPU, against the reference's outputs
+(tests/golden/dc_outputs.npz) and the oracle: MTGP32-DC on the regular
+kernels, MTGP64-DC (N - pos < 256) on the 128-word-step kernels."""
 import os
 
 import numpy as np
@@ -58,9 +59,45 @@ def test_dc32_genset_plan(mtgp, oracle, arrays):
         assert (got == want).all(), idx
 
 
-def test_dc64_rejected_on_device(mtgp):
-    g = mtgp.Generator64(record=mtgp.dc_params(64, 0), seed=1)
-    with pytest.raises(mtgp.MTGPError, match="N - pos"):
-        g.fill(0, 100)
-    with pytest.raises(mtgp.MTGPError, match="N - pos"):
-        mtgp.GenSet([mtgp.dc_params(64, 0)])
+@pytest.mark.parametrize("idx", [0, 172, 199])
+def test_dc64_fill_and_jump(mtgp, arrays, idx):
+    """MTGP64-DC sets (N = 176, N - pos down to 130) run on the 128-word-step
+    kernels: fills of every kind and device jumps against the reference."""
+    rec = mtgp.dc_params(64, idx)
+    key = "64_%d" % idx
+    g = mtgp.Generator64(record=rec, seed=1)
+    assert (g.fill(0, 1000) == arrays[key + "_init"]).all()
+    for kind in (1, 2, 3, 4):
+        got = g.fill(kind, 1000)
+        assert (got.view(np.uint8) == arrays[key + "_init_k%d" % kind].view(np.uint8)).all(), kind
+    g = mtgp.Generator64(record=rec, array=[1, 2, 3, 4])
+    assert (g.fill(0, 1000) == arrays[key + "_array"]).all()
+    for s in (1, 177, 3003, 200004):
+        g = mtgp.Generator64(record=rec, seed=0)
+        g.fill(0, 3)
+        g.fast_jump(mtgp.jump_hex(64, steps=s, record=rec))
+        assert (g.fill(0, 100) == arrays["%s_step_%d" % (key, s)]).all(), s
+
+
+def test_dc64_genset_plan(mtgp, oracle):
+    """A set of MTGP64-DC generators, each stream segmented across the GPU
+    (batched jumps on the 128-word-step jump kernel), against the oracle."""
+    from pyoracle import Oracle
+    from test_dc import _record
+    idxs = (0, 172, 199)
+    recs = [mtgp.dc_params(64, i) for i in idxs]
+    gs = mtgp.GenSet(recs)
+    gs.init_seed([3, 4, 5])
+    n = 200000
+    bufs = [mtgp.DeviceBuffer(8 * n) for _ in recs]
+    plan = gs.plan([(g, 2 if g == 1 else 0, n, 0, bufs[g]) for g in range(3)], segment=8192)
+    assert plan.stats()["items"] > 60
+    plan.launch()
+    mtgp.synchronize()
+    plan.commit()
+    for g, idx in enumerate(idxs):
+        p = Oracle.params_from_record(64, _record(recs[g]))
+        want, w = oracle.generate(p, oracle.init_state(p, 3 + g), 2 if g == 1 else 0, n)
+        got = bufs[g].to_host(want.dtype, n)
+        assert (got.view(np.uint64) == want.view(np.uint64)).all(), idx
+        assert (gs.state(g) == w).all(), idx

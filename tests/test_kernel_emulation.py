@@ -223,3 +223,68 @@ def test_init_kernel(emu, oracle, bits, mexp):
     assert emu.emu_run_init(items.ctypes.data, len(cases)) == 0
     for k, (out, w) in enumerate(zip(outs, want)):
         assert (out == w).all(), cases[k]
+
+
+@pytest.mark.parametrize("idx", [0, 172, 199])
+def test_narrow_class_items_and_jumps(emu, oracle, idx):
+    """The 128-word-step kernels (class 3: MTGP64 dynamic-creator sets, N =
+    176, N - pos down to 130) on the reference's own records: a chain of
+    segments of every kind with unaligned outputs and stepped-over gaps, and
+    a batched jump with parts, against the oracle."""
+    import json
+    import os
+    from pyoracle import Oracle
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "dc_params_digests.json")))
+    # records for 0 and 199 are in the fixture; 172 (pos 3) comes from the
+    # library's table, itself pinned to the reference digests (test_dc).
+    if "64/%d" % idx in gold["records"]:
+        rec = gold["records"]["64/%d" % idx]
+    else:
+        import mtgp_amd
+        from test_dc import _record
+        rec = _record(mtgp_amd.dc_params(64, idx))
+    p = Oracle.params_from_record(64, rec)
+    dp = pyemu.devparam(64, rec)
+    assert dp["n"][0] == 176 and dp["n"][0] - dp["pos"][0] >= 128
+    w0 = oracle.init_state(p, 11)
+    plan = [(0, 5), (2, 200), (None, 333), (1, 129), (4, 1000), (3, 1)]
+    segs = np.zeros(len(plan), pyemu.SEGMENT)
+    keep, w = [], w0
+    for k, (kind, n) in enumerate(plan):
+        if kind is None:
+            segs[k] = (0, n, 0, 0, 0)
+            _, w = oracle.generate(p, w, 0, n)
+            continue
+        raw = np.zeros((n + 40) * 8 + 256, np.uint8)
+        base = raw.ctypes.data + (-raw.ctypes.data) % 128 + 8 * (3 * k % 16)
+        segs[k] = (base, n, kind, 0, 0)
+        e, w = oracle.generate(p, w, kind, n)
+        keep.append((raw, base, n, e))
+    sin, sout = w0.copy(), np.zeros_like(w0)
+    items = np.zeros(1, pyemu.ITEM)
+    items[0]["state_in"], items[0]["state_out"] = sin.ctypes.data, sout.ctypes.data
+    items[0]["segs"], items[0]["nsegs"], items[0]["param"] = segs.ctypes.data, len(plan), 0
+    assert emu.emu_run_items_p(64, 3, 1, items.ctypes.data, 1, 1, dp.ctypes.data) == 0
+    for raw, base, n, e in keep:
+        start = base - raw.ctypes.data
+        assert (raw[start:start + n * 8].view(np.uint64) == e.view(np.uint64)).all()
+        assert (raw[start + n * 8:] == 0).all() and (raw[:start] == 0).all()
+    assert (sout == w).all()
+    # batched jump, two jumps split in two parts
+    words = 11213 // 64 + 2
+    out, _ = oracle.generate(p, oracle.init_state(p, 0), 0, 2 * 11213)
+    P, _ = oracle.bm((out & 1).astype(np.uint8), words)
+    polys = [oracle.xpow_mod(s, P) for s in (1000, 77777)]
+    coeffs = [[k for k in range(64 * len(q)) if int(q[k // 64]) >> (k % 64) & 1] for q in polys]
+    masks, nch = pyemu.jump_masks(coeffs)
+    jout = np.zeros((2, 2, 176), np.uint64)
+    jitems = np.zeros(2, pyemu.JUMP_ITEM)
+    for q in range(2):
+        jitems[q] = (w0.ctypes.data, masks.ctypes.data, jout[q].ctypes.data, 2, nch * q // 2, nch * (q + 1) // 2,
+                     0, 176 * 8, 0, (0, 0))
+    assert emu.emu_run_jumps_p(64, 3, jitems.ctypes.data, 2, 2, dp.ctypes.data) == 0
+    for b, q in enumerate(polys):
+        want = oracle.jump(p, w0, oracle.poly_to_hex(q))
+        got = jout[0, b] ^ jout[1, b]
+        assert (got[1:] == want[1:]).all() and (got[0] & p.mask) == (want[0] & p.mask)
