@@ -261,3 +261,29 @@ def test_device_fill_and_copy(mtgp):
     mtgp.synchronize()
     h = buf.to_host(np.uint32, 1100)
     assert (h[:1024] == np.arange(1024)).all() and (h[1024:1100] == 0xDEADBEEF).all()
+
+
+def test_invalid_requests_are_refused(mtgp):
+    """Planning and jump errors are reported (mtgp_last_error) and change
+    nothing: out-of-range generator or kind, NULL or misaligned outputs,
+    non-hex jump strings, an empty request list."""
+    gs = mtgp.GenSet([(32, 11213, 0), (64, 23209, 0)])
+    gs.init_seed([1, 2])
+    w0, w1 = gs.state(0), gs.state(1)
+    buf = mtgp.DeviceBuffer(1 << 16)
+    bad = [[(2, 0, 10, 0, buf)], [(0, 5, 10, 0, buf)], [(-1, 0, 10, 0, buf)], [(0, 0, 10, 0, 0)],
+           [(1, 0, 10, 0, buf.address(4))]]
+    for reqs in bad:
+        with pytest.raises(mtgp.MTGPError):
+            gs.generate(reqs)
+    with pytest.raises(mtgp.MTGPError):
+        gs.jump(0, "12xz")
+    gs.generate([])
+    assert (gs.state(0) == w0).all() and (gs.state(1) == w1).all()
+    g = mtgp.Generator32(11213, 0, seed=1)
+    w = g.window().copy()
+    with pytest.raises(mtgp.MTGPError):
+        g.fast_jump("not hex")
+    assert (g.window() == w).all()
+    with pytest.raises(mtgp.MTGPError):
+        g.fill(9, 10)
