@@ -157,11 +157,12 @@ typedef struct mtgp_request {
  * of the batch) into cells: one work item per cell, which starts from a
  * device jump-ahead (or the generator's state) and runs the pieces of the
  * consecutive requests that fall into the cell, stepping over short skips.
- * The work list lives in device memory.  mtgp_plan_launch enqueues
- * one persistent kernel launch per (word size, state size class) on
- * `stream`; it can be called again (the same numbers are produced again,
- * from the same states) and captured in a hipGraph.  mtgp_plan_commit
- * makes the states after the plan the generators' current states. */
+ * The work list lives in device memory.  mtgp_plan_launch enqueues, per
+ * (word size, state size class), a jump launch for the cells' start states
+ * and a persistent generation launch on `stream`; it can be called again
+ * (the same numbers are produced again, from the same states) and captured
+ * in a hipGraph.  mtgp_plan_commit makes the states after the plan the
+ * generators' current states. */
 typedef struct mtgp_plan mtgp_plan_t;
 mtgp_plan_t *mtgp_plan_create(mtgp_genset_t *set, const mtgp_request_t *reqs, int nreq,
                               uint64_t segment);
@@ -172,8 +173,17 @@ void mtgp_plan_destroy(mtgp_plan_t *plan);
  * numbers per lane, 128-bit stores; 1 = lane-strided, conflict-free LDS)
  * and resident workgroups per CU; -1 leaves a knob unchanged. */
 int mtgp_plan_configure(mtgp_plan_t *plan, int layout, int blocks_per_cu);
-/* Enqueues one part of a plan: 0 = the start-state jump launches, 1 = the
- * generation launches (part 1 reads what part 0 wrote: order them). */
+/* Two-phase launch order (off by default; MTGP_OVERLAP=1 changes the
+ * default): on the automatic two-size grid, mtgp_plan_launch runs the large
+ * cells' jumps, then their generation on `stream` while the small cells'
+ * jumps and generation run on a side stream of the plan, which `stream`
+ * waits for at the end.  Returns 1 if the plan now launches in two phases,
+ * 0 if it launches serially (no small cells, or off), -1 on error. */
+int mtgp_plan_set_overlap(mtgp_plan_t *plan, int on);
+/* Enqueues one part of a plan, in serial order on `stream`: 0 = the
+ * start-state jump launches, 1 = the generation launches, one per (word
+ * size, class) over every cell (part 1 reads what part 0 wrote: order
+ * them). */
 int mtgp_plan_launch_part(mtgp_plan_t *plan, int part, void *stream);
 /* Records the plan's launches (part 0 then part 1) in a hipGraph on
  * `stream` (not the null stream) and instantiates it; the graph contains
@@ -190,7 +200,8 @@ int mtgp_plan_launch_graph(mtgp_plan_t *plan, void *stream);
  * written (identically) by every trial. */
 int mtgp_plan_autotune(mtgp_plan_t *plan, void *stream, int reps, int *layout, int *blocks_per_cu,
                        float *best_ms);
-/* Work items, kernel launches, and host planning time of a plan. */
+/* Work items, kernel launches of one mtgp_plan_launch (jump and
+ * generation), and host planning time of a plan. */
 int mtgp_plan_stats(const mtgp_plan_t *plan, int *items, int *launches, double *plan_ms);
 /* plan + launch + commit + synchronize. */
 int mtgp_genset_generate(mtgp_genset_t *set, const mtgp_request_t *reqs, int nreq, void *stream);

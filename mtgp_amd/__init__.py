@@ -140,6 +140,7 @@ _sig("mtgp_plan_launch", i32, vp, vp)
 _sig("mtgp_plan_commit", i32, vp)
 _sig("mtgp_plan_destroy", None, vp)
 _sig("mtgp_plan_configure", i32, vp, i32, i32)
+_sig("mtgp_plan_set_overlap", i32, vp, i32)
 _sig("mtgp_plan_launch_part", i32, vp, i32, vp)
 _sig("mtgp_plan_stats", i32, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double))
 _sig("mtgp_plan_capture", i32, vp, vp)
@@ -511,6 +512,13 @@ class Plan:
 
     def configure(self, layout=-1, blocks_per_cu=-1):
         _check(_lib.mtgp_plan_configure(self.h, layout, blocks_per_cu), "plan_configure")
+
+    def set_overlap(self, on=True):
+        """Two-phase launch order on/off; returns True if the plan now
+        launches in two phases."""
+        r = _lib.mtgp_plan_set_overlap(self.h, 1 if on else 0)
+        _check(0 if r >= 0 else -1, "plan_set_overlap")
+        return r == 1
 
     def capture(self, stream):
         """Records the plan's launches in a hipGraph (stream: a Stream)."""

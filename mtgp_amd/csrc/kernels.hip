@@ -29,6 +29,8 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+// Angle brackets: the kernel emulation build substitutes its own copy.
+#include <gfx950_ops.hpp>
 
 namespace mtgp {
 namespace {
@@ -51,10 +53,29 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgc
 // Single-wave workgroup: orders this wave's LDS accesses across steps.
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
-__device__ __forceinline__ uint32_t temper_index(uint32_t t) {
+// The 16-entry tables (recursion tbl, tempering tmp, float tempering flt)
+// are held in LDS, contiguous.  (Replicating each to 64 entries would make
+// the lookup offset one SDWA shift, but entries 32 apart then share an LDS
+// bank: measured 2-way conflicts and a slower step.)
+constexpr int kTabRep = 16;
+constexpr int kTabWords = 3 * kTabRep;
+
+__device__ __forceinline__ uint32_t lookup(const uint32_t *t, uint32_t v) { return t[v & 15]; }
+
+// Fills tab[kTabWords] from a parameter record (DevParam's tbl, tmp and
+// flt are contiguous).
+__device__ __forceinline__ void fill_tables(uint32_t *tab, const DevParam &D) {
+    const int l = threadIdx.x;
+    if (l < kTabWords)
+        tab[l] = D.tbl[l];
+}
+
+// t ^= t >> 16; t ^= t >> 8 (mtgp32-fast.h:196-197); the low four bits
+// index the tempering tables (the mask of the second shift keeps those bits
+// and lets the compiler use a byte-select operand).
+__device__ __forceinline__ uint32_t temper_mix(uint32_t t) {
     t ^= t >> 16;
-    t ^= t >> 8;
-    return t & 15;
+    return t ^ ((t >> 8) & 0xffu);
 }
 
 // ----------------------------------------------------------------- MTGP32
@@ -65,22 +86,21 @@ struct P32 {
 };
 
 // mtgp32-fast.h:149-159
-__device__ __forceinline__ uint32_t rec32(const P32 &p, const uint32_t *tbl, uint32_t x1, uint32_t x2,
+__device__ __forceinline__ uint32_t rec32(P32 p, const uint32_t *tbl, uint32_t x1, uint32_t x2,
                                           uint32_t y) {
-    uint32_t x = (x1 & p.mask) ^ x2;
-    x ^= x << p.sh1;
-    y = x ^ (y >> p.sh2);
-    return y ^ tbl[y & 15];
+    const uint32_t x = (x1 & p.mask) ^ x2;
+    y = xor3(x, x << p.sh1, y >> p.sh2);
+    return y ^ lookup(tbl, y);
 }
 
 // Output word for each kind (mtgp32-fast.h:192-346).  All float operations
 // are exact: x - 1 and 2 - x for x in [1, 2) are representable.
 template <int KIND>
 __device__ __forceinline__ uint32_t out32(const uint32_t *tab, uint32_t r, uint32_t t) {
-    const uint32_t ti = temper_index(t);
+    const uint32_t ti = temper_mix(t);
     if (KIND == 0)
-        return r ^ tab[16 + ti];
-    const uint32_t u = (r >> 9) ^ tab[32 + ti];
+        return r ^ lookup(tab + kTabRep, ti);
+    const uint32_t u = (r >> 9) ^ lookup(tab + 2 * kTabRep, ti);
     if (KIND == 1)
         return u;
     if (KIND == 2)
@@ -93,7 +113,7 @@ __device__ __forceinline__ uint32_t out32(const uint32_t *tab, uint32_t r, uint3
 // One 256-word step from window base b (b % 4 == 0).  With OUT, writes the
 // m (<= 256) leading outputs to dst (16-byte aligned).
 template <int KIND, bool OUT, bool FULL = false>
-__device__ __forceinline__ void step32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b,
+__device__ __forceinline__ void step32(uint32_t *buf, const uint32_t *tab, P32 p, int b,
                                        gu32 *dst, uint32_t m) {
     const int l = threadIdx.x;
     const uint32_t *xp = buf + b + 4 * l;
@@ -134,7 +154,7 @@ __device__ __forceinline__ void step32(uint32_t *buf, const uint32_t *tab, const
 // and every store instruction writes 256 contiguous bytes.  No alignment
 // requirement on b or dst.
 template <int KIND, bool OUT, bool FULL = false>
-__device__ __forceinline__ void step32s(uint32_t *buf, const uint32_t *tab, const P32 &p, int b,
+__device__ __forceinline__ void step32s(uint32_t *buf, const uint32_t *tab, P32 p, int b,
                                         gu32 *dst, uint32_t m) {
     const int l = threadIdx.x;
     const uint32_t *xp = buf + b + l;
@@ -227,7 +247,7 @@ __device__ __forceinline__ void fold32(const uint32_t *buf, const gu32 *__restri
 
 // Window after jumping: acc[m] = new[l + 64m] (see file header).
 template <int NJ, int K, int NBUF>
-__device__ void jump32(uint32_t *buf, const uint32_t *tab, const P32 &p, const gu32 *__restrict__ in,
+__device__ void jump32(uint32_t *buf, const uint32_t *tab, P32 p, const gu32 *__restrict__ in,
                        const gu32 *__restrict__ poly, int poly_words, uint32_t (&acc)[NJ]) {
     const int l = threadIdx.x;
 #pragma unroll
@@ -266,7 +286,7 @@ __device__ __forceinline__ uint32_t head32(uint64_t out, uint64_t count) {
 // Window base for a segment with head h: layout 0 needs (b + h) % 4 == 0;
 // both need room for one more step.  Moves the window when either fails.
 template <int NJ, int NBUF, int LAYOUT>
-__device__ __forceinline__ int place32(uint32_t *buf, const P32 &p, int b, uint32_t h) {
+__device__ __forceinline__ int place32(uint32_t *buf, P32 p, int b, uint32_t h) {
     const int to = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
     const bool misaligned = LAYOUT == 0 && (((uint32_t)b + h) & 3u) != 0;
     if (misaligned || b + p.n + kStep > NBUF) {
@@ -278,7 +298,7 @@ __device__ __forceinline__ int place32(uint32_t *buf, const P32 &p, int b, uint3
 
 // Steps the window over `count` numbers without output; returns the new base.
 template <int NJ, int NBUF>
-__device__ int advance32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b, uint64_t count) {
+__device__ __forceinline__ int advance32(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t count) {
     for (uint64_t s = count / kStep; s > 0; s--) {
         if (b + p.n + kStep > NBUF) {
             relocate32<NJ>(buf, b, p.n);
@@ -304,7 +324,7 @@ __device__ int advance32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b
 // Writes `count` numbers of KIND to out from the window at base b (placed
 // by place32 for head h); returns the window base after the last number.
 template <int KIND, int NJ, int NBUF, int LAYOUT>
-__device__ int generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b, uint32_t h,
+__device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint32_t h,
                           gu32 *__restrict__ out, uint64_t count) {
     const int l = threadIdx.x;
     gu32 *dst = out;
@@ -362,8 +382,8 @@ __device__ int generate32(uint32_t *buf, const uint32_t *tab, const P32 &p, int 
 
 // One segment (Segment, common.hpp) from window base b; returns the new base.
 template <int NJ, int NBUF, int LAYOUT>
-__device__ int segment32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b, uint64_t out, uint64_t count,
-                         int kind) {
+__device__ __forceinline__ int segment32_body(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t out,
+                                              uint64_t count, int kind) {
     if (count == 0)
         return b;
     if (out == 0)
@@ -380,6 +400,23 @@ __device__ int segment32(uint32_t *buf, const uint32_t *tab, const P32 &p, int b
     }
 }
 
+// The small state classes (NJ <= 6) inline the segment into the kernel, so
+// that LDS offsets fold into the instructions; the large ones call an
+// out-of-line copy (fully inlined, their kernels spill registers).
+template <int NJ, int NBUF, int LAYOUT>
+__device__ __noinline__ int segment32_call(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t out,
+                                           uint64_t count, int kind) {
+    return segment32_body<NJ, NBUF, LAYOUT>(buf, tab, p, b, out, count, kind);
+}
+template <int NJ, int NBUF, int LAYOUT>
+__device__ __forceinline__ int segment32(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t out,
+                                         uint64_t count, int kind) {
+    if constexpr (NJ <= 6)
+        return segment32_body<NJ, NBUF, LAYOUT>(buf, tab, p, b, out, count, kind);
+    else
+        return segment32_call<NJ, NBUF, LAYOUT>(buf, tab, p, b, out, count, kind);
+}
+
 template <int NMAX, int K, int LAYOUT>
 __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict__ items, int n_items,
                                                           const DevParam *__restrict__ params,
@@ -387,7 +424,7 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
     constexpr int NJ = (NMAX + kWave - 1) / kWave;
     constexpr int NBUF = K * kStep + kWave * NJ + kWave;
     __shared__ __attribute__((aligned(16))) uint32_t buf[NBUF];
-    __shared__ uint32_t tab[48];
+    __shared__ uint32_t tab[kTabWords];
     const int l = threadIdx.x;
     uint32_t it = blockIdx.x;
     while (it < (uint32_t)n_items) {
@@ -399,8 +436,7 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
         p.sh1 = D.sh1;
         p.sh2 = D.sh2;
         p.mask = D.mask_lo;
-        if (l < 48)
-            tab[l] = D.tbl[l];  // tbl, tmp, flt are contiguous
+        fill_tables(tab, D);
         gu32 *state_out = reinterpret_cast<gu32 *>(I.state_out);
         const gu32 *state_in = reinterpret_cast<const gu32 *>(I.state_in);
         wave_sync();
@@ -459,16 +495,15 @@ struct P64 {
 
 // mtgp64-fast.h:148-169: (rh, rl) = rec(x1, x2, y); only the high half of
 // a table entry is non-zero.
-__device__ __forceinline__ void rec64(const P64 &p, const uint32_t *tbl, uint32_t x1h, uint32_t x1l,
+__device__ __forceinline__ void rec64(P64 p, const uint32_t *tbl, uint32_t x1h, uint32_t x1l,
                                       uint32_t x2h, uint32_t x2l, uint32_t yh, uint32_t yl, uint32_t &rh,
                                       uint32_t &rl) {
-    uint32_t xh = (x1h & p.mask_hi) ^ x2h;
-    uint32_t xl = (x1l & p.mask_lo) ^ x2l;
-    xh ^= xh << p.sh1;
-    xl ^= xl << p.sh1;
-    const uint32_t nh = xl ^ (yh >> p.sh2);
-    const uint32_t nl = xh ^ (yl >> p.sh2);
-    rh = nh ^ tbl[nl & 15];
+    const uint32_t xh = (x1h & p.mask_hi) ^ x2h;
+    const uint32_t xl = (x1l & p.mask_lo) ^ x2l;
+    // halves shifted separately, then crossed (xh ^= xh << sh1, ...)
+    const uint32_t nh = xor3(xl, xl << p.sh1, yh >> p.sh2);
+    const uint32_t nl = xor3(xh, xh << p.sh1, yl >> p.sh2);
+    rh = nh ^ lookup(tbl, nl);
     rl = nl;
 }
 
@@ -476,14 +511,14 @@ __device__ __forceinline__ void rec64(const P64 &p, const uint32_t *tbl, uint32_
 template <int KIND>
 __device__ __forceinline__ void out64(const uint32_t *tab, uint32_t rh, uint32_t rl, uint32_t tl, uint32_t &oh,
                                       uint32_t &ol) {
-    const uint32_t ti = temper_index(tl);
+    const uint32_t ti = temper_mix(tl);
     if (KIND == 0) {
-        oh = rh ^ tab[16 + ti];
+        oh = rh ^ lookup(tab + kTabRep, ti);
         ol = rl;
         return;
     }
     uint32_t lo = (rl >> 12) | (rh << 20);
-    const uint32_t hi = (rh >> 12) ^ tab[32 + ti];
+    const uint32_t hi = (rh >> 12) ^ lookup(tab + 2 * kTabRep, ti);
     if (KIND == 4)
         lo |= 1u;
     if (KIND == 1) {
@@ -502,7 +537,7 @@ __device__ __forceinline__ void out64(const uint32_t *tab, uint32_t rh, uint32_t
 }
 
 template <int KIND, bool OUT, bool FULL = false>
-__device__ __forceinline__ void step64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b,
+__device__ __forceinline__ void step64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b,
                                        gu32 *dst, uint32_t m) {
     const int l = threadIdx.x;
     const int q = b + 4 * l;
@@ -552,7 +587,7 @@ __device__ __forceinline__ void step64(uint32_t *bh, uint32_t *bl, const uint32_
 // Lane-strided variant of step64 (see step32s); outputs are 8-byte stores,
 // 512 contiguous bytes per instruction.
 template <int KIND, bool OUT, bool FULL = false, int WPL = 4>
-__device__ __forceinline__ void step64s(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b,
+__device__ __forceinline__ void step64s(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b,
                                         gu32 *dst, uint32_t m) {
     const int l = threadIdx.x;
     const int q = b + l;
@@ -648,7 +683,7 @@ __device__ __forceinline__ void fold64(const uint32_t *bh, const uint32_t *bl, c
 }
 
 template <int NJ, int K, int NBUF>
-__device__ void jump64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p,
+__device__ void jump64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p,
                        const gu64 *__restrict__ in, const gu32 *__restrict__ poly, int poly_words,
                        uint32_t (&ah)[NJ], uint32_t (&al)[NJ]) {
     const int l = threadIdx.x;
@@ -688,7 +723,7 @@ __device__ __forceinline__ uint32_t head64(uint64_t out, uint64_t count) {
 // states with N - pos < 256 (the MTGP64 dynamic-creator sets, N = 176),
 // lane-strided layout only.
 template <int NJ, int NBUF, int LAYOUT, int WPL = 4>
-__device__ __forceinline__ int place64(uint32_t *bh, uint32_t *bl, const P64 &p, int b, uint32_t h) {
+__device__ __forceinline__ int place64(uint32_t *bh, uint32_t *bl, P64 p, int b, uint32_t h) {
     const int to = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
     const bool misaligned = LAYOUT == 0 && (((uint32_t)b + h) & 3u) != 0;
     if (misaligned || b + p.n + kWave * WPL > NBUF) {
@@ -699,7 +734,7 @@ __device__ __forceinline__ int place64(uint32_t *bh, uint32_t *bl, const P64 &p,
 }
 
 template <int NJ, int NBUF, int WPL = 4>
-__device__ int advance64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b, uint64_t count) {
+__device__ __forceinline__ int advance64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b, uint64_t count) {
     constexpr int W = kWave * WPL;
     for (uint64_t s = count / W; s > 0; s--) {
         if (b + p.n + W > NBUF) {
@@ -724,7 +759,7 @@ __device__ int advance64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const 
 }
 
 template <int KIND, int NJ, int NBUF, int LAYOUT, int WPL = 4>
-__device__ int generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b, uint32_t h,
+__device__ __forceinline__ int generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b, uint32_t h,
                           gu32 *__restrict__ out, uint64_t count) {
     static_assert(LAYOUT == 1 || WPL == 4, "128-word steps are lane-strided");
     constexpr int W = kWave * WPL;
@@ -779,9 +814,9 @@ __device__ int generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const
     return b;
 }
 
-template <int NJ, int NBUF, int LAYOUT, int WPL = 4>
-__device__ int segment64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const P64 &p, int b, uint64_t out,
-                         uint64_t count, int kind) {
+template <int NJ, int NBUF, int LAYOUT, int WPL>
+__device__ __forceinline__ int segment64_body(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b,
+                                              uint64_t out, uint64_t count, int kind) {
     if (count == 0)
         return b;
     if (out == 0)
@@ -798,6 +833,23 @@ __device__ int segment64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, const 
     }
 }
 
+// Inlined for the 2-words-per-lane class, an out-of-line call otherwise:
+// inlined, the 4-words-per-lane kernels need about 180 VGPRs (2 waves per
+// SIMD) against about 100 with the call.
+template <int NJ, int NBUF, int LAYOUT, int WPL>
+__device__ __noinline__ int segment64_call(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b,
+                                           uint64_t out, uint64_t count, int kind) {
+    return segment64_body<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, out, count, kind);
+}
+template <int NJ, int NBUF, int LAYOUT, int WPL = 4>
+__device__ __forceinline__ int segment64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b,
+                                         uint64_t out, uint64_t count, int kind) {
+    if constexpr (WPL == 2)
+        return segment64_body<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, out, count, kind);
+    else
+        return segment64_call<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, out, count, kind);
+}
+
 template <int NMAX, int K, int LAYOUT, int WPL = 4>
 __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict__ items, int n_items,
                                                           const DevParam *__restrict__ params,
@@ -806,7 +858,7 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
     constexpr int NBUF = K * kWave * WPL + kWave * NJ + kWave;
     __shared__ __attribute__((aligned(16))) uint32_t bh[NBUF];
     __shared__ __attribute__((aligned(16))) uint32_t bl[NBUF];
-    __shared__ uint32_t tab[48];
+    __shared__ uint32_t tab[kTabWords];
     const int l = threadIdx.x;
     uint32_t it = blockIdx.x;
     while (it < (uint32_t)n_items) {
@@ -819,8 +871,7 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         p.sh2 = D.sh2;
         p.mask_hi = D.mask_hi;
         p.mask_lo = D.mask_lo;
-        if (l < 48)
-            tab[l] = D.tbl[l];
+        fill_tables(tab, D);
         gu64 *state_out = reinterpret_cast<gu64 *>(I.state_out);
         const gu64 *state_in = reinterpret_cast<const gu64 *>(I.state_in);
         wave_sync();
@@ -937,7 +988,7 @@ __global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restr
     constexpr int NBUF = K * kStep + kWave * NJ + kWave;
     static_assert(B * 16 == 64 && QMAX == 16, "one 64-bit mask word per residue");
     __shared__ __attribute__((aligned(16))) uint32_t buf[NBUF];
-    __shared__ uint32_t tab[48];
+    __shared__ uint32_t tab[kTabWords];
     const int l = threadIdx.x;
     uint32_t it = blockIdx.x;
     while (it < (uint32_t)n_items) {
@@ -949,8 +1000,7 @@ __global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restr
         p.sh1 = D.sh1;
         p.sh2 = D.sh2;
         p.mask = D.mask_lo;
-        if (l < 48)
-            tab[l] = D.tbl[l];
+        fill_tables(tab, D);
         const gu32 *in = reinterpret_cast<const gu32 *>(I.state_in);
         const gu64c *mk = reinterpret_cast<const gu64c *>(I.masks);
         wave_sync();
@@ -1021,7 +1071,7 @@ __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restr
     constexpr int NBUF = kJumpChunk + kWave * NJ + kWave;
     __shared__ __attribute__((aligned(16))) uint32_t bh[NBUF];
     __shared__ __attribute__((aligned(16))) uint32_t bl[NBUF];
-    __shared__ uint32_t tab[48];
+    __shared__ uint32_t tab[kTabWords];
     const int l = threadIdx.x;
     uint32_t it = blockIdx.x;
     while (it < (uint32_t)n_items) {
@@ -1034,8 +1084,7 @@ __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restr
         p.sh2 = D.sh2;
         p.mask_hi = D.mask_hi;
         p.mask_lo = D.mask_lo;
-        if (l < 48)
-            tab[l] = D.tbl[l];
+        fill_tables(tab, D);
         const gu64 *in = reinterpret_cast<const gu64 *>(I.state_in);
         const gu64c *mk = reinterpret_cast<const gu64c *>(I.masks);
         wave_sync();

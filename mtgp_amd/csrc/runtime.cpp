@@ -113,6 +113,13 @@ Context &context() {
         if (v >= 0 && v < 1)
             c->segment_split = v;
     }
+    if (const char *e = getenv("MTGP_OVERLAP"))
+        c->overlap = atoi(e) != 0;
+    if (const char *e = getenv("MTGP_JUMP_WAVES")) {
+        const int v = atoi(e);
+        if (v > 0 && v <= 64)
+            c->jump_waves_per_cu = v;
+    }
     if (const char *e = getenv("MTGP_SEGMENT_ROUNDS")) {
         const double v = atof(e);
         if (v >= 0.25 && v <= 64)
@@ -834,6 +841,13 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         if (g.bits == 64 && c.class_of(g.slot) == 3)
             jump_mode = 1;
     const bool masks = jump_mode == 1;
+    // Phase of a cell: 1 for the small cells of a two-size grid (batched
+    // jumps only), else 0.  Phase 1 has launches of its own, which the
+    // two-phase launch order runs on a side stream.
+    const bool phased = masks && frac > 0;
+    auto phase_of = [&](int g, size_t k) {
+        return (phased && cells[(size_t)g][k].start >= Bg[(size_t)g]) ? 1 : 0;
+    };
     std::vector<std::vector<std::vector<uint32_t>>> polys((size_t)G);
     std::vector<std::vector<std::vector<uint16_t>>> m16((size_t)G);
     auto cached = [&](int g, size_t k, bool cheap) {
@@ -896,7 +910,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     });
     // Jump groups (jump_mode 1): up to kJumpBatch cells of one generator.
     struct Group {
-        int g;
+        int g, phase;
         std::vector<size_t> cells;
         size_t mask_off;  // byte offset into the mask array
         int nchunks;
@@ -906,28 +920,33 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     std::vector<uint32_t> poly_host;
     if (masks) {
         for (int g = 0; g < G; g++) {
-            std::vector<size_t> need;
-            for (size_t k = 0; k < cells[(size_t)g].size(); k++)
-                if (cells[(size_t)g][k].start > 0)
-                    need.push_back(k);
-            for (size_t i0 = 0; i0 < need.size(); i0 += kJumpBatch) {
-                Group gr;
-                gr.g = g;
-                gr.cells.assign(need.begin() + (long)i0, need.begin() + (long)std::min(need.size(), i0 + kJumpBatch));
-                std::vector<const std::vector<uint16_t> *> ps;
-                for (size_t k : gr.cells)
-                    ps.push_back(&m16[(size_t)g][k]);
-                gr.mask_off = mask_host.size() * 8;
-                gr.nchunks = append_jump_masks16(ps, mask_host);
-                groups.push_back(gr);
+            for (int ph = 0; ph < 2; ph++) {
+                std::vector<size_t> need;
+                for (size_t k = 0; k < cells[(size_t)g].size(); k++)
+                    if (cells[(size_t)g][k].start > 0 && phase_of(g, k) == ph)
+                        need.push_back(k);
+                for (size_t i0 = 0; i0 < need.size(); i0 += kJumpBatch) {
+                    Group gr;
+                    gr.g = g;
+                    gr.phase = ph;
+                    gr.cells.assign(need.begin() + (long)i0,
+                                    need.begin() + (long)std::min(need.size(), i0 + kJumpBatch));
+                    std::vector<const std::vector<uint16_t> *> ps;
+                    for (size_t k : gr.cells)
+                        ps.push_back(&m16[(size_t)g][k]);
+                    gr.mask_off = mask_host.size() * 8;
+                    gr.nchunks = append_jump_masks16(ps, mask_host);
+                    groups.push_back(gr);
+                }
             }
         }
     }
     // Split each group's chunks over `parts` waves so the jump launch has
-    // about four waves per SIMD; the partial windows are XORed on load.
+    // about jump_waves_per_cu waves per CU; the partial windows are XORed
+    // on load.
     int parts = 1;
     if (!groups.empty()) {
-        const int target = c.cus * 16;
+        const int target = c.cus * c.jump_waves_per_cu;
         parts = std::max(1, std::min<int>((target + (int)groups.size() - 1) / (int)groups.size(),
                                           std::max(1, opt.max_parts)));
     }
@@ -955,7 +974,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             ji.param = ref.slot;
             ji.state_bytes = (int32_t)sbytes;
             jitems.push_back(ji);
-            jgroup.push_back(group_of(ref));
+            jgroup.push_back(2 * group_of(ref) + gr.phase);
         }
         arena_bytes += align256((size_t)P * nb * sbytes);
     }
@@ -1021,12 +1040,12 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             list.push_back(it);
             arena_slot.push_back(slot);
             cost.push_back(n + (it.poly != UINT64_MAX ? 200000u : 0u));
-            group.push_back(group_of(gr));
+            group.push_back(2 * group_of(gr) + phase_of(g, k));
             if (it.state_out)
                 commits.push_back(g);
         }
     }
-    // Longest first within each (word size, class) group.
+    // By (word size, class) group, phase 0 before phase 1, longest first.
     std::vector<int> order(list.size());
     for (size_t i = 0; i < order.size(); i++)
         order[i] = (int)i;
@@ -1095,7 +1114,8 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     if (opt.sync)
         hip_check(hipStreamSynchronize(opt.stream), "plan upload sync");
     // Launches: one jump launch and one generation launch per (word size,
-    // class) group.
+    // class) group and phase, and the same merged over the two phases (one
+    // contiguous range per group: the serial order).
     auto group_launches = [&](const std::vector<int> &grp, bool jump) {
         std::vector<Launch> out;
         for (size_t i = 0; i < grp.size();) {
@@ -1104,8 +1124,9 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             while (j < grp.size() && grp[j] == gi)
                 j++;
             Launch L;
-            L.bits = gi >= 3 ? 64 : 32;
-            L.cls = gi >= 3 ? gi - 3 : gi;
+            L.phase = gi % 2;
+            L.bits = gi / 2 >= 3 ? 64 : 32;
+            L.cls = gi / 2 >= 3 ? gi / 2 - 3 : gi / 2;
             L.first = (int)i;
             L.count = (int)(j - i);
             if (jump) {
@@ -1128,8 +1149,19 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         sorted_group.push_back(group[(size_t)i]);
     launches = group_launches(sorted_group, false);
     jlaunches = group_launches(jgroup, true);
-    if (2 * (launches.size() + jlaunches.size()) > 64)
+    auto no_phase = [](std::vector<int> v) {
+        for (int &x : v)
+            x &= ~1;
+        return v;
+    };
+    mlaunches = group_launches(no_phase(sorted_group), false);
+    mjlaunches = group_launches(no_phase(jgroup), true);
+    if (2 * (launches.size() + jlaunches.size() + mlaunches.size() + mjlaunches.size()) > 64)
         throw Error("too many launches in one plan");
+    has_phase1 = false;
+    for (const Launch &L : launches)
+        has_phase1 |= L.phase == 1;
+    overlap = opt.overlap >= 0 ? opt.overlap != 0 : c.overlap;
     segment_used = S;
     plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -1145,21 +1177,50 @@ void Plan::reserve(size_t work_bytes, size_t jstates_bytes) {
 }
 
 void Plan::launch(hipStream_t s) {
-    launch_part(0, s);
-    launch_part(1, s);
+    if (!two_phase()) {
+        launch_part(0, s);
+        launch_part(1, s);
+        return;
+    }
+    if (!side) {
+        hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        hip_check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+        hip_check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreateWithFlags");
+    }
+    launch_phase(0, 0, s);
+    hip_check(hipEventRecord(ev_fork, s), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(side, ev_fork, 0), "hipStreamWaitEvent");
+    launch_phase(0, 1, side);
+    launch_phase(1, 0, s);
+    launch_phase(1, 1, side);
+    hip_check(hipEventRecord(ev_join, side), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(s, ev_join, 0), "hipStreamWaitEvent");
 }
 
-void Plan::launch_part(int part, hipStream_t s) {
-    const size_t jq = 2 * launches.size();
-    for (size_t k = 0; part == 0 && k < jlaunches.size(); k++) {
-        const Launch &L = jlaunches[k];
+void Plan::launch_part(int part, hipStream_t s) { run_launches(part, -1, s); }
+
+void Plan::launch_phase(int part, int phase, hipStream_t s) { run_launches(part, phase, s); }
+
+// Work-queue counter pairs: [launches | jlaunches | mlaunches | mjlaunches].
+void Plan::run_launches(int part, int phase, hipStream_t s) {
+    const bool merged = phase < 0;
+    const auto &J = merged ? mjlaunches : jlaunches;
+    const auto &I = merged ? mlaunches : launches;
+    const size_t iq = merged ? 2 * (launches.size() + jlaunches.size()) : 0;
+    const size_t jq = iq + 2 * I.size();
+    for (size_t k = 0; part == 0 && k < J.size(); k++) {
+        const Launch &L = J[k];
+        if (!merged && L.phase != phase)
+            continue;
         hip_check(launch_jumps(L.bits, L.cls, d_jitems + L.first, L.count, ctx->d_params, d_queue + jq + 2 * k, L.grid,
                                L.dyn_lds, s),
                   "jump kernel launch");
     }
-    for (size_t k = 0; part == 1 && k < launches.size(); k++) {
-        const Launch &L = launches[k];
-        hip_check(launch_items(L.bits, L.cls, L.layout, d_items + L.first, L.count, ctx->d_params, d_queue + 2 * k,
+    for (size_t k = 0; part == 1 && k < I.size(); k++) {
+        const Launch &L = I[k];
+        if (!merged && L.phase != phase)
+            continue;
+        hip_check(launch_items(L.bits, L.cls, L.layout, d_items + L.first, L.count, ctx->d_params, d_queue + iq + 2 * k,
                                L.grid, L.dyn_lds, s),
                   "kernel launch");
     }
@@ -1167,7 +1228,8 @@ void Plan::launch_part(int part, hipStream_t s) {
 
 void Plan::configure(int layout, int bpc) {
     drop_graph();
-    for (auto &L : launches) {
+    for (auto *Ls : {&launches, &mlaunches})
+    for (auto &L : *Ls) {
         if (layout >= 0) {
             if (layout >= kLayouts)
                 throw Error("unknown layout");
@@ -1271,6 +1333,14 @@ float Plan::autotune(hipStream_t s, int reps, int *best_layout, int *best_bpc) {
 
 void Plan::free_device() {
     drop_graph();
+    if (ev_fork)
+        (void)hipEventDestroy(ev_fork);
+    if (ev_join)
+        (void)hipEventDestroy(ev_join);
+    if (side)
+        (void)hipStreamDestroy(side);
+    ev_fork = ev_join = nullptr;
+    side = nullptr;
     if (d_work)
         (void)hipFree(d_work);
     if (d_jstates)
@@ -1878,6 +1948,13 @@ int mtgp_plan_configure(mtgp_plan_t *p, int layout, int blocks_per_cu) {
     API_END(-1)
 }
 
+int mtgp_plan_set_overlap(mtgp_plan_t *p, int on) {
+    API_BEGIN
+    p->plan.set_overlap(on != 0);
+    return p->plan.two_phase() ? 1 : 0;
+    API_END(-1)
+}
+
 int mtgp_plan_capture(mtgp_plan_t *p, void *stream) {
     API_BEGIN
     p->plan.capture((hipStream_t)stream);
@@ -1904,8 +1981,9 @@ int mtgp_plan_autotune(mtgp_plan_t *p, void *stream, int reps, int *layout, int 
 int mtgp_plan_stats(const mtgp_plan_t *p, int *items, int *launches, double *plan_ms) {
     if (items)
         *items = (int)p->plan.items.size();
-    if (launches)
-        *launches = (int)p->plan.launches.size();
+    if (launches)  // kernel launches of one mtgp_plan_launch
+        *launches = p->plan.two_phase() ? (int)(p->plan.launches.size() + p->plan.jlaunches.size())
+                                      : (int)(p->plan.mlaunches.size() + p->plan.mjlaunches.size());
     if (plan_ms)
         *plan_ms = p->plan.plan_ms;
     return 0;

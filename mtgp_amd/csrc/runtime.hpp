@@ -56,6 +56,16 @@ struct Context {
     // 0.7 measured best on the batched MTGP32-11213 workload (generation
     // 4.03-4.06 ms vs 4.21-4.25 ms with 1.5 rounds of equal cells).
     double segment_split = 0.7;
+    // Two-phase launches of two-size grids (MTGP_OVERLAP=1).  Off by
+    // default: on the batched MTGP32-11213 workload the side stream's jumps
+    // time-share the generation kernel's VALU rather than hide behind its
+    // stores (4.8 ms per step against 4.6 ms serial).
+    bool overlap = false;
+    // Batched jumps are split into partial jumps (chunk ranges, XORed when
+    // the generation kernel loads the start window) until a jump launch has
+    // about this many waves per CU: a jump launch is one round of waves, so
+    // its duration is one wave's share of the chunks.
+    int jump_waves_per_cu = 16;
     struct Custom {
         DevParam rec;
         unsigned char sha1[20];
@@ -148,6 +158,7 @@ struct PlanOptions {
     bool sync = true;              // wait for the upload before returning
     int jump_mode = -1;            // 0 fused, 1 batched, -1 the plan's default
     int max_parts = 8;             // waves a batched jump is split over, at most
+    int overlap = -1;              // two-phase launch (see Plan): 1 on, 0 off, -1 default
 };
 
 // A list of device Items plus the launches that run them.
@@ -158,10 +169,21 @@ struct PlanOptions {
 // state for the first cell) and then its segments in order -- pieces of
 // requests, or numbers stepped over -- so consecutive requests need no jump
 // between them.
+//
+// Two-phase launch (overlap): with the two-size grid, the large cells (one
+// per resident workgroup, the first round) form phase 0 and the small cells
+// phase 1.  launch() runs phase 0's jumps, then phase 0's generation on the
+// caller's stream while phase 1's jumps, then phase 1's generation, run on
+// a side stream; the caller's stream waits for the side stream at the end.
+// Phase 1's jumps (VALU work) so overlap generation (store-bound), and phase
+// 1's generation fills phase 0's tail.  launch_part runs the merged launches
+// (one jump and one generation launch per class over both phases: the
+// serial order).
 struct Plan {
     struct Launch {
         int bits, cls, layout, first, count, grid;
         size_t dyn_lds;
+        int phase;
     };
     Context *ctx = nullptr;
     // Start states of cells: 0 = jump inside the generation kernel
@@ -173,6 +195,9 @@ struct Plan {
     std::vector<Launch> launches;
     std::vector<JumpItem> jitems;
     std::vector<Launch> jlaunches;  // layout unused
+    // The same work merged over the phases: one range per (word size,
+    // class), run by launch_part (serial order).
+    std::vector<Launch> mlaunches, mjlaunches;
     std::vector<int> commits;       // generators whose state moves
     // Device copies: one grow-only workspace [items | segs | polys | masks |
     // jump items], the jumped-state arena and the work-queue counters.
@@ -189,6 +214,15 @@ struct Plan {
     uint64_t numbers = 0;  // numbers generated or stepped over
     int jumps = 0;         // cells that start with a jump
     double plan_ms = 0;
+    bool has_phase1 = false;  // the small cells form launches of their own
+    bool overlap = false;
+    // Batched jumps are split into partial jumps (chunk ranges, XORed when
+    // the generation kernel loads the start window) until a jump launch has
+    // about this many waves per CU: a jump launch is one round of waves, so
+    // its duration is one wave's share of the chunks.
+    int jump_waves_per_cu = 16;     // launch() uses the two-phase order
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     void build(Context &c, const std::vector<GenRef> &gens, const std::vector<mtgp_request_t> &reqs,
                const PlanOptions &opt);
@@ -203,6 +237,13 @@ struct Plan {
     void reserve(size_t work_bytes, size_t jstates_bytes);
     void launch(hipStream_t s);
     void launch_part(int part, hipStream_t s);  // 0 jumps, 1 generation
+    void launch_phase(int part, int phase, hipStream_t s);
+    void run_launches(int part, int phase, hipStream_t s);  // phase -1: merged
+    bool two_phase() const { return overlap && has_phase1; }
+    void set_overlap(bool on) {
+        drop_graph();
+        overlap = on;
+    }
     // Overrides the step layout and/or the resident workgroups per CU of
     // every launch (-1 keeps the current value).
     void configure(int layout, int blocks_per_cu);

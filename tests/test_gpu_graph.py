@@ -65,3 +65,44 @@ def test_autotune_keeps_results_exact(mtgp, oracle):
     plan.launch()
     mtgp.synchronize()
     _check(bufs, want, "after autotune layout %d bpc %d" % (layout, bpc))
+
+
+def test_two_phase_launch_serial_and_graph(mtgp, oracle):
+    """A plan on the automatic two-size grid launches in two phases (the
+    large cells' jumps, then their generation beside the small cells'
+    jumps and generation on a side stream).  Its outputs equal the oracle
+    eagerly, in serial order (launch_part) and replayed from a graph
+    captured across the two streams."""
+    gs, reqs, want, bufs = _plan_case(mtgp, oracle, count=200000)
+    plan = gs.plan(reqs)  # automatic grid
+    assert plan.set_overlap(True)
+    st = plan.stats()
+    # three (word size, class) groups: 3 jump + 3 generation launches
+    # merged; more when the small cells form a second phase
+    assert st["launches"] > 6, st
+    junk = np.full(max(e.nbytes for e in want), 0x5A, np.uint8)
+
+    def scribble():
+        for b, e in zip(bufs, want):
+            b.from_host(junk[:e.nbytes])
+        mtgp.synchronize()
+
+    stream = mtgp.Stream()
+    scribble()
+    plan.launch(stream)
+    stream.synchronize()
+    _check(bufs, want, "two-phase eager")
+    scribble()
+    plan.launch_part(0, stream)
+    plan.launch_part(1, stream)
+    stream.synchronize()
+    _check(bufs, want, "serial")
+    plan.capture(stream)
+    other = mtgp.Generator32(11213, 9, seed=3)
+    for rep in range(3):
+        scribble()
+        other.fill(0, 50000)
+        mtgp.synchronize()
+        plan.launch_graph(stream)
+        stream.synchronize()
+        _check(bufs, want, "two-phase replay %d" % rep)
