@@ -58,6 +58,18 @@ const char *mtgp_last_error(void);
 int mtgp_set_device(int device);
 /* Compute units and HBM bytes of the current device. */
 int mtgp_device_info(int *compute_units, size_t *hbm_bytes);
+/* The planner's tuning knobs (defaults measured on MI355X; each also set
+ * by an environment variable at start-up): "segment_split" (MTGP_SEGMENT_
+ * SPLIT, share of each stream in first-round cells of the two-size grid),
+ * "segment_rounds" (MTGP_SEGMENT_ROUNDS), "two_size_rho" (MTGP_TWO_SIZE_RHO,
+ * largest jump cost / one-round cell size that keeps the two-size grid),
+ * "jump_rounds_k" and "jump_rounds_min" (MTGP_JUMP_ROUNDS_K/_MIN: beyond
+ * it, cells for k / rho rounds, at least min), "jump_waves_per_cu"
+ * (MTGP_JUMP_WAVES), "overlap" (MTGP_OVERLAP, default two-phase launch).
+ * They apply to plans created afterwards.  -1 for an unknown name or a
+ * value out of range. */
+int mtgp_tuning_set(const char *name, double value);
+int mtgp_tuning_get(const char *name, double *value);
 void *mtgp_dev_malloc(size_t bytes);
 int mtgp_dev_free(void *ptr);
 int mtgp_memcpy_htod(void *dst, const void *src, size_t bytes);

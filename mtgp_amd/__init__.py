@@ -141,6 +141,8 @@ _sig("mtgp_plan_commit", i32, vp)
 _sig("mtgp_plan_destroy", None, vp)
 _sig("mtgp_plan_configure", i32, vp, i32, i32)
 _sig("mtgp_plan_set_overlap", i32, vp, i32)
+_sig("mtgp_tuning_set", i32, C.c_char_p, C.c_double)
+_sig("mtgp_tuning_get", i32, C.c_char_p, C.POINTER(C.c_double))
 _sig("mtgp_plan_launch_part", i32, vp, i32, vp)
 _sig("mtgp_plan_stats", i32, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double))
 _sig("mtgp_plan_capture", i32, vp, vp)
@@ -214,6 +216,36 @@ def device_info():
     cus, hbm = C.c_int(), C.c_size_t()
     _check(_lib.mtgp_device_info(C.byref(cus), C.byref(hbm)), "mtgp_device_info")
     return cus.value, hbm.value
+
+
+def tuning_set(name, value):
+    """Sets a planner tuning knob (mtgp_tuning_set) for plans created later."""
+    _check(_lib.mtgp_tuning_set(name.encode(), float(value)), "mtgp_tuning_set")
+
+
+def tuning_get(name):
+    v = C.c_double()
+    _check(_lib.mtgp_tuning_get(name.encode(), C.byref(v)), "mtgp_tuning_get")
+    return v.value
+
+
+class tuning:
+    """Context manager: `with tuning(two_size_rho=1e9): ...` sets knobs and
+    restores their previous values on exit."""
+
+    def __init__(self, **knobs):
+        self.knobs, self.saved = knobs, {}
+
+    def __enter__(self):
+        for k, v in self.knobs.items():
+            self.saved[k] = tuning_get(k)
+            tuning_set(k, v)
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.saved.items():
+            tuning_set(k, v)
+        return False
 
 
 class DeviceBuffer:

@@ -115,6 +115,12 @@ Context &context() {
     }
     if (const char *e = getenv("MTGP_OVERLAP"))
         c->overlap = atoi(e) != 0;
+    if (const char *e = getenv("MTGP_TWO_SIZE_RHO"))
+        c->two_size_rho = atof(e);
+    if (const char *e = getenv("MTGP_JUMP_ROUNDS_MIN"))
+        c->jump_rounds_min = std::max(0.05, atof(e));
+    if (const char *e = getenv("MTGP_JUMP_ROUNDS_K"))
+        c->jump_rounds_k = std::max(0.01, atof(e));
     if (const char *e = getenv("MTGP_JUMP_WAVES")) {
         const int v = atoi(e);
         if (v > 0 && v <= 64)
@@ -693,16 +699,31 @@ void grow_pinned(char *&p, size_t &cap, size_t need) {
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 }  // namespace
 
-void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int slots, uint64_t &S1, uint64_t &S2,
-               double &frac) {
+double jump_cost_numbers(int mexp, int n) { return 0.026 * (double)mexp * (double)n; }
+
+void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_gens, int slots, double jump_cost,
+               uint64_t &S1, uint64_t &S2, double &frac) {
+    const double usable = std::max(1.0, (double)slots - (double)n_requests);
+    auto round256 = [](double v) { return std::max<uint64_t>(((uint64_t)v + 255) & ~255ull, 4096); };
+    const double rho = jump_cost / std::max(1.0, (double)total_numbers / usable);
+    if (rho > c.two_size_rho) {
+        // One round of cells on about r of the slots, as a whole number of
+        // cells per CU (an uneven count leaves some CUs one wave heavier,
+        // whose waves then finish last); each generator's last cell is on
+        // average half full.
+        const double r = std::min(1.0, std::max(c.jump_rounds_min, c.jump_rounds_k / rho));
+        const double per_cu = std::max(1.0, std::round(usable * r / (double)c.cus));
+        const double cells = std::max(1.0, per_cu * (double)c.cus - 0.5 * (double)n_gens);
+        frac = 0;
+        S1 = S2 = round256(std::ceil((double)total_numbers / cells));
+        return;
+    }
     frac = c.segment_split;
     if (frac <= 0 || frac >= 1) {
         frac = 0;
         S1 = S2 = auto_segment(c, total_numbers, n_requests, slots);
         return;
     }
-    const double usable = std::max(1.0, (double)slots - (double)n_requests);
-    auto round256 = [](double v) { return std::max<uint64_t>(((uint64_t)v + 255) & ~255ull, 4096); };
     S1 = round256(std::ceil(frac * (double)total_numbers / usable));
     S2 = round256(std::ceil((1 - frac) * (double)total_numbers / usable));
 }
@@ -733,6 +754,21 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         const int b = g.bits == 64, cls = c.class_of(g.slot);
         slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][c.layout[b][cls]]));
     }
+    // Numbers-weighted mean jump cost of the plan's generators.
+    double jcost = 0;
+    int active = 0;
+    {
+        std::vector<uint64_t> per((size_t)G, 0);
+        for (const auto &r : reqs)
+            per[(size_t)r.gen] += r.count;
+        double w = 0;
+        for (int g = 0; g < G; g++) {
+            jcost += (double)per[(size_t)g] * jump_cost_numbers(gens[(size_t)g].mexp, gens[(size_t)g].n);
+            w += (double)per[(size_t)g];
+            active += per[(size_t)g] > 0;
+        }
+        jcost = w > 0 ? jcost / w : 0;
+    }
     uint64_t S1 = 0, S2 = 0;
     double frac = 0;
     if (opt.cache)
@@ -740,7 +776,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     else if (opt.segment)
         S1 = S2 = opt.segment;
     else
-        auto_grid(c, all, nonempty, slots, S1, S2, frac);
+        auto_grid(c, all, nonempty, active, slots, jcost, S1, S2, frac);
     const uint64_t S = S1;
     // Requests as runs of stream positions per generator.
     struct Run {
@@ -1558,6 +1594,70 @@ int mtgp_device_info(int *cus, size_t *hbm) {
         *cus = c.cus;
     if (hbm)
         *hbm = c.hbm_bytes;
+    return 0;
+    API_END(-1)
+}
+
+namespace {
+// The planner's tuning knobs by name (each also read from an environment
+// variable at start-up).
+double *tuning_knob(Context &c, const std::string &name) {
+    if (name == "segment_split")
+        return &c.segment_split;
+    if (name == "segment_rounds")
+        return &c.segment_rounds;
+    if (name == "two_size_rho")
+        return &c.two_size_rho;
+    if (name == "jump_rounds_k")
+        return &c.jump_rounds_k;
+    if (name == "jump_rounds_min")
+        return &c.jump_rounds_min;
+    return nullptr;
+}
+}  // namespace
+
+int mtgp_tuning_set(const char *name, double value) {
+    API_BEGIN
+    Context &c = context();
+    const std::string n = name ? name : "";
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (n == "overlap") {
+        c.overlap = value != 0;
+        return 0;
+    }
+    if (n == "jump_waves_per_cu") {
+        if (value < 1 || value > 64)
+            throw Error("jump_waves_per_cu outside [1, 64]");
+        c.jump_waves_per_cu = (int)value;
+        return 0;
+    }
+    double *k = tuning_knob(c, n);
+    if (!k)
+        throw Error("unknown tuning knob '" + n + "'");
+    if (!(value >= 0) || (n == "segment_split" && value >= 1) || (n == "segment_rounds" && value < 0.25) ||
+        (n == "jump_rounds_min" && (value <= 0 || value > 1)) || (n == "jump_rounds_k" && value <= 0))
+        throw Error("value out of range for '" + n + "'");
+    *k = value;
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_tuning_get(const char *name, double *value) {
+    API_BEGIN
+    Context &c = context();
+    const std::string n = name ? name : "";
+    std::lock_guard<std::mutex> lk(c.mu);
+    double v;
+    if (n == "overlap")
+        v = c.overlap ? 1 : 0;
+    else if (n == "jump_waves_per_cu")
+        v = c.jump_waves_per_cu;
+    else if (double *k = tuning_knob(c, n))
+        v = *k;
+    else
+        throw Error("unknown tuning knob '" + n + "'");
+    if (value)
+        *value = v;
     return 0;
     API_END(-1)
 }

@@ -56,6 +56,16 @@ struct Context {
     // 0.7 measured best on the batched MTGP32-11213 workload (generation
     // 4.03-4.06 ms vs 4.21-4.25 ms with 1.5 rounds of equal cells).
     double segment_split = 0.7;
+    // Jump-cost-aware grid: with rho = (jump cost) / (cell size of one round
+    // over every slot), rho <= two_size_rho keeps the grid above; beyond it
+    // cells are sized for jump_rounds_k / rho rounds (at least
+    // jump_rounds_min, at most 1).  Measured on MI355X: MTGP64-23209
+    // (rho 0.41) best at 0.5 round, 3.66 TB/s against 2.60 TB/s with the
+    // two-size grid; MTGP64-44497 (rho 1.1) best at 0.35 round, 2.97 against
+    // 1.45 TB/s; MTGP32-11213 (rho 0.09) best with the two-size grid.
+    double two_size_rho = 0.15;
+    double jump_rounds_k = 0.2;
+    double jump_rounds_min = 0.35;
     // Two-phase launches of two-size grids (MTGP_OVERLAP=1).  Off by
     // default: on the batched MTGP32-11213 workload the side stream's jumps
     // time-share the generation kernel's VALU rather than hide behind its
@@ -113,10 +123,20 @@ struct GenRef {
 };
 
 uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, int slots);
+// Cost of one batched jump-ahead, in numbers the same kernel class
+// generates in the same chip time: a jump is about MEXP * N / 2 word XORs
+// (on two planes for MTGP64, whose numbers also take twice as long to
+// generate).  Measured on MI355X at about 0.026 * MEXP * N numbers for
+// MTGP32-11213 and MTGP64-23209 alike.
+double jump_cost_numbers(int mexp, int n);
 // Cell sizes for automatic segmentation: S1 for the first `frac` of each
-// generator's positions, S2 after (frac = 0: one size).
-void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int slots, uint64_t &S1, uint64_t &S2,
-               double &frac);
+// generator's positions, S2 after (frac = 0: one size).  `jump_cost` is the
+// numbers-weighted mean jump_cost_numbers of the plan's generators: when a
+// jump costs much less than a cell of one round over every slot, the
+// two-size grid (two rounds); otherwise one size, with fewer cells the
+// dearer a jump is.
+void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_gens, int slots, double jump_cost,
+               uint64_t &S1, uint64_t &S2, double &frac);
 
 // Jump polynomials x^pos mod P kept per parameter slot for positions on a
 // grid of `cell` numbers (pos = k * cell): table T1[k] = x^(k cell) for

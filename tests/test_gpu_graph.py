@@ -74,7 +74,9 @@ def test_two_phase_launch_serial_and_graph(mtgp, oracle):
     eagerly, in serial order (launch_part) and replayed from a graph
     captured across the two streams."""
     gs, reqs, want, bufs = _plan_case(mtgp, oracle, count=200000)
-    plan = gs.plan(reqs)  # automatic grid
+    # automatic two-size grid, kept whatever the jump cost of this small case
+    with mtgp.tuning(two_size_rho=1e9):
+        plan = gs.plan(reqs)
     assert plan.set_overlap(True)
     st = plan.stats()
     # three (word size, class) groups: 3 jump + 3 generation launches
@@ -106,3 +108,27 @@ def test_two_phase_launch_serial_and_graph(mtgp, oracle):
         plan.launch_graph(stream)
         stream.synchronize()
         _check(bufs, want, "two-phase replay %d" % rep)
+
+
+def test_tuning_knobs_and_jump_cost_grid(mtgp, oracle):
+    """Tuning knobs round-trip and reject bad input; a plan whose jumps are
+    dear next to its cells (the jump-cost rule) uses fewer cells than the
+    two-size grid, and both grids give the oracle's numbers."""
+    assert mtgp.tuning_get("two_size_rho") > 0
+    with pytest.raises(mtgp.MTGPError):
+        mtgp.tuning_set("no_such_knob", 1)
+    with pytest.raises(mtgp.MTGPError):
+        mtgp.tuning_set("segment_split", 1.5)
+    # 24 M numbers: large enough that the grids differ (cells of at least
+    # 4096 numbers)
+    gs, reqs, want, bufs = _plan_case(mtgp, oracle, count=2000000)
+    with mtgp.tuning(two_size_rho=1e9):
+        two = gs.plan(reqs)
+    cheap = gs.plan(reqs)
+    assert cheap.stats()["items"] < two.stats()["items"]
+    for plan in (two, cheap):
+        for b, e in zip(bufs, want):
+            b.from_host(np.full(e.nbytes, 0x5A, np.uint8))
+        plan.launch()
+        mtgp.synchronize()
+        _check(bufs, want, "grid with %d items" % plan.stats()["items"])
