@@ -146,10 +146,10 @@ def run_gpu(args):
         plan.launch(stream)
     stream.synchronize()
     M.synchronize()
-    # Whole job, one step = plan.launch: the jumps of the large (first-round)
-    # cells, then their generation on `stream` while the small cells' jumps
-    # and generation run on the plan's side stream, joined back into
-    # `stream` at the end of the step.
+    # Whole job, one step = plan.launch: the jump launch for the cells'
+    # start states, then the persistent generation launch (or, with the
+    # plan's two-phase order switched on, the large cells' jumps and
+    # generation beside the small cells' on a side stream).
     evw = [M.Event() for _ in range(args.steps + 1)]
     evw[0].record(stream)
     for k in range(args.steps):
@@ -330,8 +330,8 @@ def main():
         "numbers_per_s": round(r["numbers"] * args.steps / (r["total_ms"] * 1e-3), 1),
         "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
         "config": {"workload": "batched MTGP32-11213: 128 parameter sets, uneven lengths 2^20..2^26 "
-                               "uint32 + close_open float32 each; per step the plan's two-phase launch (large "
-                               "cells' jumps, then their generation beside the small cells' jumps and generation)",
+                               "uint32 + close_open float32 each; per step one jump launch (cell start states) + "
+                               "one persistent generation launch",
                    "generators": r["G"], "output_bytes": r["out_bytes"], "numbers": r["numbers"],
                    "work_items": r["stats"]["items"], "launches_per_step": r["stats"]["launches"],
                    "plan_ms": round(r["stats"]["plan_ms"], 1), "compute_units": r["cus"],

@@ -38,24 +38,30 @@ struct Context {
     int blocks_per_cu[2][4][2] = {};
     int jump_blocks_per_cu[2][4] = {};
     int layout[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
-    // Resident workgroups per CU used by default (measured best for the
-    // generation kernels: 16 of the 32 the occupancy limit allows).
-    int default_bpc = 16;
+    // Resident workgroups per CU used by default.  The generation kernel is
+    // held back by the board's power budget (its clock drops to 1.6-2.0 GHz
+    // under full store traffic), so more waves do not generate faster but
+    // do mean more cells, hence more jumps: on the batched MTGP32-11213
+    // workload 12 per CU measured 4.49 ms per step (generation 3.90 ms)
+    // against 4.60-4.63 ms at 16 and 4.71 ms at 8.
+    int default_bpc = 12;
     // Resident workgroups per CU for generation launches per [bits==64][class]
     // (default_bpc until the autotuner measures better).
-    int bpc[2][4] = {{16, 16, 16, 16}, {16, 16, 16, 16}};
+    int bpc[2][4] = {{12, 12, 12, 12}, {12, 12, 12, 12}};
     // Automatic segmentation cuts the work of a plan into about this many
     // segments per resident workgroup slot: more rounds balance the tail
     // better, each extra segment costs one jump-ahead (1.5 measured best for
-    // uneven MTGP32-11213 batches at 16 workgroups per CU).
+    // uneven MTGP32-11213 batches at 16 workgroups per CU; used when the
+    // two-size grid is off).
     double segment_rounds = 1.5;
     // Two-size automatic cells: the first `segment_split` of each
     // generator's positions in cells sized for one round over all slots, the
     // rest in cells sized for a second round (so the last round's cells are
     // small and fill the tail); 0 = equal cells, segment_rounds rounds.
-    // 0.7 measured best on the batched MTGP32-11213 workload (generation
-    // 4.03-4.06 ms vs 4.21-4.25 ms with 1.5 rounds of equal cells).
-    double segment_split = 0.7;
+    // 0.8 measured best on the batched MTGP32-11213 workload at 12
+    // workgroups per CU (4.38-4.41 ms per step; 0.7: 4.43-4.52, 0.85:
+    // 4.55, 0.9: 4.71; equal cells at 1.5 rounds were slower still).
+    double segment_split = 0.8;
     // Jump-cost-aware grid: with rho = (jump cost) / (cell size of one round
     // over every slot), rho <= two_size_rho keeps the grid above; beyond it
     // cells are sized for jump_rounds_k / rho rounds (at least
