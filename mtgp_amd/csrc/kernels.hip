@@ -83,6 +83,7 @@ __device__ __forceinline__ uint32_t temper_mix(uint32_t t) {
 struct P32 {
     int n, pos, sh1, sh2;
     uint32_t mask;
+    int pm1;  // pos - 1: the tempering word's offset, kept in a scalar register
 };
 
 // mtgp32-fast.h:149-159
@@ -158,7 +159,7 @@ __device__ __forceinline__ void step32s(uint32_t *buf, const uint32_t *tab, P32 
                                         gu32 *dst, uint32_t m) {
     const int l = threadIdx.x;
     const uint32_t *xp = buf + b + l;
-    const uint32_t *yp = xp + p.pos - 1;
+    const uint32_t *yp = xp + p.pm1;
     uint32_t r[4], t[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -174,12 +175,11 @@ __device__ __forceinline__ void step32s(uint32_t *buf, const uint32_t *tab, P32 
 #pragma unroll
         for (int i = 0; i < 4; i++)
             o[i] = out32<KIND>(tab, r[i], t[i]);
+        // the m leading outputs: lanes past them fall outside the window
+        const store_window_t w = store_window((uint64_t)(uintptr_t)dst, FULL ? 4u * kStep : 4u * m);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const uint32_t q = (uint32_t)(l + kWave * i);
-            if (FULL || q < m)
-                dst[q] = o[i];
-        }
+        for (int i = 0; i < 4; i++)
+            store_u32(w, 4u * (uint32_t)(l + kWave * i), o[i]);
     }
 }
 
@@ -433,6 +433,7 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
         P32 p;
         p.n = D.n;
         p.pos = D.pos;
+        p.pm1 = D.pos - 1;
         p.sh1 = D.sh1;
         p.sh2 = D.sh2;
         p.mask = D.mask_lo;
@@ -997,6 +998,7 @@ __global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restr
         P32 p;
         p.n = D.n;
         p.pos = D.pos;
+        p.pm1 = D.pos - 1;
         p.sh1 = D.sh1;
         p.sh2 = D.sh2;
         p.mask = D.mask_lo;
