@@ -61,6 +61,7 @@ int mtgp_device_info(int *compute_units, size_t *hbm_bytes);
 /* The planner's tuning knobs (defaults measured on MI355X; each also set
  * by an environment variable at start-up): "segment_split" (MTGP_SEGMENT_
  * SPLIT, share of each stream in first-round cells of the two-size grid),
+ * "small_rounds" (MTGP_SMALL_ROUNDS, small cells per slot of that grid),
  * "segment_rounds" (MTGP_SEGMENT_ROUNDS), "two_size_rho" (MTGP_TWO_SIZE_RHO,
  * largest jump cost / one-round cell size that keeps the two-size grid),
  * "jump_rounds_k" and "jump_rounds_min" (MTGP_JUMP_ROUNDS_K/_MIN: beyond

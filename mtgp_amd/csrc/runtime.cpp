@@ -115,6 +115,11 @@ Context &context() {
     }
     if (const char *e = getenv("MTGP_OVERLAP"))
         c->overlap = atoi(e) != 0;
+    if (const char *e = getenv("MTGP_SMALL_ROUNDS")) {
+        const double v = atof(e);
+        if (v >= 0.1 && v <= 8)
+            c->small_rounds = v;
+    }
     if (const char *e = getenv("MTGP_TWO_SIZE_RHO"))
         c->two_size_rho = atof(e);
     if (const char *e = getenv("MTGP_JUMP_ROUNDS_MIN"))
@@ -725,7 +730,7 @@ void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_g
         return;
     }
     S1 = round256(std::ceil(frac * (double)total_numbers / usable));
-    S2 = round256(std::ceil((1 - frac) * (double)total_numbers / usable));
+    S2 = round256(std::ceil((1 - frac) * (double)total_numbers / (usable * c.small_rounds)));
 }
 
 void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<mtgp_request_t> &reqs,
@@ -1606,6 +1611,8 @@ double *tuning_knob(Context &c, const std::string &name) {
         return &c.segment_split;
     if (name == "segment_rounds")
         return &c.segment_rounds;
+    if (name == "small_rounds")
+        return &c.small_rounds;
     if (name == "two_size_rho")
         return &c.two_size_rho;
     if (name == "jump_rounds_k")
@@ -1635,7 +1642,8 @@ int mtgp_tuning_set(const char *name, double value) {
     if (!k)
         throw Error("unknown tuning knob '" + n + "'");
     if (!(value >= 0) || (n == "segment_split" && value >= 1) || (n == "segment_rounds" && value < 0.25) ||
-        (n == "jump_rounds_min" && (value <= 0 || value > 1)) || (n == "jump_rounds_k" && value <= 0))
+        (n == "jump_rounds_min" && (value <= 0 || value > 1)) || (n == "jump_rounds_k" && value <= 0) ||
+        (n == "small_rounds" && (value < 0.1 || value > 8)))
         throw Error("value out of range for '" + n + "'");
     *k = value;
     return 0;

@@ -62,6 +62,9 @@ struct Context {
     // workgroups per CU (4.38-4.41 ms per step; 0.7: 4.43-4.52, 0.85:
     // 4.55, 0.9: 4.71; equal cells at 1.5 rounds were slower still).
     double segment_split = 0.8;
+    // Small cells of the two-size grid per resident slot (their count is
+    // about small_rounds * slots; each costs a jump).
+    double small_rounds = 1.0;
     // Jump-cost-aware grid: with rho = (jump cost) / (cell size of one round
     // over every slot), rho <= two_size_rho keeps the grid above; beyond it
     // cells are sized for jump_rounds_k / rho rounds (at least
