@@ -181,6 +181,83 @@ def run_gpu(args):
                 plan_s=time.time() - t0, exact=exact)
 
 
+def _time_plan(M, plan, stream, reps=3):
+    """(jump_ms, generate_ms) of a plan: mean over `reps` launches after one
+    warm-up, HIP events on the stream the kernels run on."""
+    plan.launch(stream)
+    stream.synchronize()
+    ev = [M.Event() for _ in range(3)]
+    j = g = 0.0
+    for _ in range(reps):
+        ev[0].record(stream)
+        plan.launch_part(0, stream)
+        ev[1].record(stream)
+        plan.launch_part(1, stream)
+        ev[2].record(stream)
+        stream.synchronize()
+        j += ev[0].elapsed_ms(ev[1])
+        g += ev[1].elapsed_ms(ev[2])
+    return j / reps, g / reps
+
+
+def run_configs(M):
+    """The other configurations, timed like the main one (start-state jump
+    launch + generation launch per step, inputs resident): 1 (one stream),
+    3 (MTGP64 batches per MEXP) and 4 (4096 substreams)."""
+    stream = M.Stream()
+    out = {}
+
+    def line(nbytes, jump_ms, gen_ms, items, **extra):
+        d = {"output_bytes": nbytes, "jump_ms": round(jump_ms, 4), "generate_ms": round(gen_ms, 4),
+             "ms": round(jump_ms + gen_ms, 4), "gbs": round(nbytes / ((jump_ms + gen_ms) * 1e-3) / 1e9, 2),
+             "generate_gbs": round(nbytes / (gen_ms * 1e-3) / 1e9, 2), "work_items": items}
+        d.update(extra)
+        return d
+    # 1: set 0 of MTGP32-11213, mtgp32_init(1234), 2^24 uint32 then 2^24 [1,2) floats
+    n = 1 << 24
+    gs = M.GenSet([(32, 11213, 0)])
+    gs.init_seed([1234])
+    buf = M.DeviceBuffer(8 * n)
+    plan = gs.plan([(0, M.KIND_INT, n, 0, buf.address(0)), (0, M.KIND_CLOSE1_OPEN2, n, 0, buf.address(4 * n))])
+    out["1"] = line(8 * n, *_time_plan(M, plan, stream), plan.stats()["items"],
+                    workload="MTGP32-11213 set 0, seed 1234: 2^24 uint32 then 2^24 float [1,2), one stream")
+    del plan, buf, gs
+    # 3: MTGP64 sets 0..63 per MEXP, init_by_array with 4-word splitmix64 keys,
+    # 2^24 uint64 then 2^24 double [0,1) each
+    for mexp in (23209, 44497):
+        G = 64
+        gs = M.GenSet([(64, mexp, i) for i in range(G)])
+        gs.init_array([splitmix64(1000 * mexp + i, 4) for i in range(G)])
+        buf = M.DeviceBuffer(G * 16 * n)
+        reqs = []
+        for i in range(G):
+            reqs.append((i, M.KIND_INT, n, 0, buf.address(16 * n * i)))
+            reqs.append((i, M.KIND_CLOSE_OPEN, n, 0, buf.address(16 * n * i + 8 * n)))
+        plan = gs.plan(reqs)
+        out["3-%d" % mexp] = line(G * 16 * n, *_time_plan(M, plan, stream), plan.stats()["items"],
+                                  workload="MTGP64-%d sets 0..63: 2^24 uint64 then 2^24 double [0,1) each" % mexp)
+        del plan, buf, gs
+    # 4: 4096 substreams i * 3^162 of set 0, seed 777, 2^20 uint32 each
+    S, count = 4096, 1 << 20
+    base = M.GenSet([(32, 11213, 0)])
+    base.init_seed([777])
+    w0 = base.state(0)
+    gs = M.GenSet([(32, 11213, 0)] * S)
+    js = M.jump_hex(32, 11213, 0, 3 ** 162)
+    M.synchronize()
+    t = time.time()
+    gs.substreams(w0, js)
+    M.synchronize()
+    setup_ms = (time.time() - t) * 1e3
+    buf = M.DeviceBuffer(S * 4 * count)
+    plan = gs.plan([(i, M.KIND_INT, count, 0, buf.address(4 * count * i)) for i in range(S)])
+    out["4"] = line(S * 4 * count, *_time_plan(M, plan, stream), plan.stats()["items"],
+                    substream_setup_ms=round(setup_ms, 1),
+                    workload="4096 substreams i*3^162 of MTGP32-11213 set 0 (seed 777), 2^20 uint32 each; "
+                             "substream_setup_ms (host polynomials + device jumps, wall clock) is outside ms")
+    return out
+
+
 def spot_check(M, buf, seeds, lengths, n=1 << 16):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Oracle
@@ -296,11 +373,17 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=20000.0, help="serving requests per second")
     ap.add_argument("--serving-requests", type=int, default=12000)
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--configs", action="store_true",
+                    help="time the other configurations (1, 3, 4) and print their JSON instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every generator of the workload fits one MI355X and the "
               "path has no cross-GPU step; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.configs:
+        import mtgp_amd as M
+        print(json.dumps({"configs": run_configs(M)}), flush=True)
+        return
     if args.only_serving:
         import mtgp_amd as M
         print(json.dumps({"serving": run_serving(M, args.serving_requests, args.serving_rate)}), flush=True)

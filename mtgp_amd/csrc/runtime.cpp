@@ -1176,7 +1176,14 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
                 L.dyn_lds = 0;
             } else {
                 L.layout = c.layout[L.bits == 64][L.cls];
-                const int bpc = std::min(c.bpc[L.bits == 64][L.cls], c.blocks_per_cu[L.bits == 64][L.cls][L.layout]);
+                const int cap = c.blocks_per_cu[L.bits == 64][L.cls][L.layout];
+                int bpc = std::min(c.bpc[L.bits == 64][L.cls], cap);
+                // Up to half a round more cells than slots (e.g. one cell per
+                // generator for thousands of generators) run as one round
+                // when the occupancy limit allows: a partial second round
+                // would leave most waves idle.
+                if (L.count > c.cus * bpc && L.count <= c.cus * cap && 2 * L.count <= 3 * c.cus * bpc)
+                    bpc = (L.count + c.cus - 1) / c.cus;
                 L.grid = std::min(L.count, c.cus * bpc);
                 L.dyn_lds = c.lds_reserve(kernel_fn(L.bits, L.cls, L.layout), bpc);
             }
