@@ -67,8 +67,8 @@ int mtgp_device_info(int *compute_units, size_t *hbm_bytes);
  * "jump_rounds_k" and "jump_rounds_min" (MTGP_JUMP_ROUNDS_K/_MIN: beyond
  * it, cells for k / rho rounds, at least min), "jump_waves_per_cu"
  * (MTGP_JUMP_WAVES), "overlap" (MTGP_OVERLAP, default two-phase launch).
- * They apply to plans created afterwards.  -1 for an unknown name or a
- * value out of range. */
+ * They apply to plans created afterwards; set them while no other thread
+ * is creating a plan.  -1 for an unknown name or a value out of range. */
 int mtgp_tuning_set(const char *name, double value);
 int mtgp_tuning_get(const char *name, double *value);
 void *mtgp_dev_malloc(size_t bytes);

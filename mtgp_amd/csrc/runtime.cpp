@@ -154,7 +154,7 @@ void select_device(int dev) {
 }
 
 size_t Context::lds_reserve(const void *fn, int per_cu) {
-    // Off unless asked for: measured neutral at the default 16 per CU.
+    // Off unless asked for: measured neutral at 16 per CU.
     const char *e = getenv("MTGP_LDS_CAP");
     if (!e || atoi(e) == 0)
         return 0;
