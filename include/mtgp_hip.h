@@ -66,7 +66,9 @@ int mtgp_device_info(int *compute_units, size_t *hbm_bytes);
  * largest jump cost / one-round cell size that keeps the two-size grid),
  * "jump_rounds_k" and "jump_rounds_min" (MTGP_JUMP_ROUNDS_K/_MIN: beyond
  * it, cells for k / rho rounds, at least min), "jump_waves_per_cu"
- * (MTGP_JUMP_WAVES), "overlap" (MTGP_OVERLAP, default two-phase launch).
+ * (MTGP_JUMP_WAVES), "jump_max_parts" (MTGP_JUMP_MAX_PARTS, most partial
+ * jumps a jump is split into), "overlap" (MTGP_OVERLAP, default two-phase
+ * launch).
  * They apply to plans created afterwards; set them while no other thread
  * is creating a plan.  -1 for an unknown name or a value out of range. */
 int mtgp_tuning_set(const char *name, double value);

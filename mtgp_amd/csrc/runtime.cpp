@@ -126,6 +126,11 @@ Context &context() {
         c->jump_rounds_min = std::max(0.05, atof(e));
     if (const char *e = getenv("MTGP_JUMP_ROUNDS_K"))
         c->jump_rounds_k = std::max(0.01, atof(e));
+    if (const char *e = getenv("MTGP_JUMP_MAX_PARTS")) {
+        const int v = atoi(e);
+        if (v > 0 && v <= 64)
+            c->jump_max_parts = v;
+    }
     if (const char *e = getenv("MTGP_JUMP_WAVES")) {
         const int v = atoi(e);
         if (v > 0 && v <= 64)
@@ -985,11 +990,20 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     // Split each group's chunks over `parts` waves so the jump launch has
     // about jump_waves_per_cu waves per CU; the partial windows are XORed
     // on load.
+    // The split stays within one round of the jump kernel's occupancy (the
+    // smallest of the plan's classes): a partial second round would double
+    // the launch's duration.
     int parts = 1;
     if (!groups.empty()) {
-        const int target = c.cus * c.jump_waves_per_cu;
-        parts = std::max(1, std::min<int>((target + (int)groups.size() - 1) / (int)groups.size(),
-                                          std::max(1, opt.max_parts)));
+        const int ng = (int)groups.size();
+        int occ = 64;
+        for (const Group &gr : groups) {
+            const GenRef &ref = gens[(size_t)gr.g];
+            occ = std::min(occ, c.jump_blocks_per_cu[ref.bits == 64][c.class_of(ref.slot)]);
+        }
+        const int want = (c.cus * c.jump_waves_per_cu + ng - 1) / ng;
+        const int fit = (c.cus * occ) / ng;
+        parts = std::max(1, std::min({want, fit, std::max(1, opt.max_parts > 0 ? opt.max_parts : c.jump_max_parts)}));
     }
     std::vector<int> jgroup;
     jitems.clear();
@@ -1645,6 +1659,12 @@ int mtgp_tuning_set(const char *name, double value) {
         c.jump_waves_per_cu = (int)value;
         return 0;
     }
+    if (n == "jump_max_parts") {
+        if (value < 1 || value > 64)
+            throw Error("jump_max_parts outside [1, 64]");
+        c.jump_max_parts = (int)value;
+        return 0;
+    }
     double *k = tuning_knob(c, n);
     if (!k)
         throw Error("unknown tuning knob '" + n + "'");
@@ -1667,6 +1687,8 @@ int mtgp_tuning_get(const char *name, double *value) {
         v = c.overlap ? 1 : 0;
     else if (n == "jump_waves_per_cu")
         v = c.jump_waves_per_cu;
+    else if (n == "jump_max_parts")
+        v = c.jump_max_parts;
     else if (double *k = tuning_knob(c, n))
         v = *k;
     else
