@@ -1244,11 +1244,7 @@ void Plan::launch(hipStream_t s) {
         launch_part(1, s);
         return;
     }
-    if (!side) {
-        hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
-        hip_check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreateWithFlags");
-        hip_check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreateWithFlags");
-    }
+    ensure_side();
     launch_phase(0, 0, s);
     hip_check(hipEventRecord(ev_fork, s), "hipEventRecord");
     hip_check(hipStreamWaitEvent(side, ev_fork, 0), "hipStreamWaitEvent");
@@ -1257,6 +1253,14 @@ void Plan::launch(hipStream_t s) {
     launch_phase(1, 1, side);
     hip_check(hipEventRecord(ev_join, side), "hipEventRecord");
     hip_check(hipStreamWaitEvent(s, ev_join, 0), "hipStreamWaitEvent");
+}
+
+void Plan::ensure_side() {
+    if (side)
+        return;
+    hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    hip_check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+    hip_check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreateWithFlags");
 }
 
 void Plan::launch_part(int part, hipStream_t s) { run_launches(part, -1, s); }
@@ -1317,6 +1321,8 @@ void Plan::capture(hipStream_t s) {
     if (!s)
         throw Error("capture needs a stream other than the null stream");
     drop_graph();
+    if (two_phase())
+        ensure_side();  // created outside the capture
     hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
         launch(s);

@@ -279,6 +279,7 @@ struct Plan {
     void launch_phase(int part, int phase, hipStream_t s);
     void run_launches(int part, int phase, hipStream_t s);  // phase -1: merged
     bool two_phase() const { return overlap && has_phase1; }
+    void ensure_side();  // the side stream and fork/join events
     void set_overlap(bool on) {
         drop_graph();
         overlap = on;

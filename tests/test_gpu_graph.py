@@ -108,6 +108,16 @@ def test_two_phase_launch_serial_and_graph(mtgp, oracle):
         plan.launch_graph(stream)
         stream.synchronize()
         _check(bufs, want, "two-phase replay %d" % rep)
+    # captured before any eager launch: the side stream is made outside
+    # the capture
+    with mtgp.tuning(two_size_rho=1e9):
+        fresh = gs.plan(reqs)
+    assert fresh.set_overlap(True)
+    fresh.capture(stream)
+    scribble()
+    fresh.launch_graph(stream)
+    stream.synchronize()
+    _check(bufs, want, "two-phase graph captured first")
 
 
 def test_tuning_knobs_and_jump_cost_grid(mtgp, oracle):
