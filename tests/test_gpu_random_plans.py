@@ -1,11 +1,14 @@
 """Randomised plans against the oracle: mixed word sizes and state classes,
-every output kind, short and long skips (stepped over or jumped), zero-count
+table and dynamic-creator sets (records), every output kind, short and
+long skips (stepped over or jumped), zero-count
 requests, automatic and explicit cell grids, both step layouts, the
 two-size grid forced on small batches, the two-phase launch order, and a
 re-launch before the commit.  Every output number and every final window is
 compared bit for bit."""
 import numpy as np
 import pytest
+from pyoracle import Oracle
+from test_dc import _record
 
 pytestmark = pytest.mark.gpu
 
@@ -25,11 +28,28 @@ def test_random_plan(mtgp, oracle, seed):
         bits, mexp = SETS[int(rng.choice(len(SETS), p=[0.3, 0.15, 0.1, 0.25, 0.12, 0.08]))]
         specs.append((bits, mexp, int(rng.integers(0, 128))))
     seeds = [int(v) for v in rng.integers(1, 2**32, G)]
-    gs = mtgp.GenSet(specs)
+    # About a third of the cases build the set from parameter records, half
+    # of them dynamic-creator sets (MTGP64-DC runs on the 128-word-step
+    # class); a separate generator keeps the other draws as they were.
+    rrng = np.random.default_rng(5000 + seed)
+    recs = None
+    if rrng.random() < 0.35:
+        recs = []
+        for g, (bits, mexp, idx) in enumerate(specs):
+            if rrng.random() < 0.5:
+                bits = int(rrng.choice([32, 64]))
+                recs.append(mtgp.dc_params(bits, int(rrng.integers(0, 200))))
+                specs[g] = (bits, 11213, -1)
+            else:
+                recs.append(mtgp.params(bits, mexp, idx))
+    gs = mtgp.GenSet(recs if recs is not None else specs)
     gs.init_seed(seeds)
     st = []
     for g, (bits, mexp, idx) in enumerate(specs):
-        p = oracle.params(bits, mexp, idx)
+        if idx < 0:
+            p = Oracle.params_from_record(bits, _record(recs[g]))
+        else:
+            p = oracle.params(bits, mexp, idx)
         st.append([p, oracle.init_state(p, seeds[g] if bits == 64 else seeds[g] & 0xFFFFFFFF), None])
     reqs, expect, offs = [], [], []
     off = 0
@@ -50,7 +70,9 @@ def test_random_plan(mtgp, oracle, seed):
         total += count + (skip if skip < 1 << 30 else 0)
         if skip >= 1 << 30:
             if P is None:
-                P = st[g][2] = oracle.hex_to_poly(mtgp.charpoly_hex(bits, mexp, idx), mexp // 64 + 2)
+                hexp = mtgp.charpoly_hex(bits, record=recs[g]) if recs is not None else \
+                    mtgp.charpoly_hex(bits, mexp, idx)
+                P = st[g][2] = oracle.hex_to_poly(hexp, mexp // 64 + 2)
             w = oracle.jump(p, w, oracle.poly_to_hex(oracle.xpow_mod(skip, P)))
         elif skip:
             _, w = oracle.generate(p, w, 0, skip)
@@ -73,8 +95,8 @@ def test_random_plan(mtgp, oracle, seed):
     overlap = plan.set_overlap(rng.random() < 0.5)
     if rng.random() < 0.3:
         plan.configure(layout=int(rng.integers(0, 2)))
-    what = "seed %d: %d gens %s, %d requests, segment %d, two-size %s, two-phase %s, items %d" % (
-        seed, G, specs, len(reqs), segment, force_two_size, overlap, plan.stats()["items"])
+    what = "seed %d: %d gens %s (records %s), %d requests, segment %d, two-size %s, two-phase %s, items %d" % (
+        seed, G, specs, recs is not None, len(reqs), segment, force_two_size, overlap, plan.stats()["items"])
     buf.from_host(np.full(off + 64, 0xA5, np.uint8))
     plan.launch()
     mtgp.synchronize()
