@@ -744,6 +744,11 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     ctx = &c;
     const int G = (int)gens.size();
     uint64_t all = 0;
+    int nonempty = 0;
+    // Generators the requests name, in first-use order, with their counts.
+    std::vector<int> named;
+    std::vector<uint64_t> per((size_t)G, 0);
+    std::vector<char> seen((size_t)G, 0);
     for (const auto &r : reqs) {
         if (r.gen < 0 || r.gen >= G)
             throw Error("request for generator " + std::to_string(r.gen) + " outside the set");
@@ -754,31 +759,27 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         if (r.count && ((uintptr_t)r.out % (uintptr_t)elem_bytes(gens[(size_t)r.gen].bits)))
             throw Error("output buffer not aligned to its element size");
         all += r.count;
-    }
-    int nonempty = 0;
-    for (const auto &r : reqs)
         nonempty += r.count > 0;
-    // Resident workgroups of the default configuration of the largest class.
-    int slots = 0;
-    for (const auto &g : gens) {
-        const int b = g.bits == 64, cls = c.class_of(g.slot);
-        slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][c.layout[b][cls]]));
-    }
-    // Numbers-weighted mean jump cost of the plan's generators.
-    double jcost = 0;
-    int active = 0;
-    {
-        std::vector<uint64_t> per((size_t)G, 0);
-        for (const auto &r : reqs)
-            per[(size_t)r.gen] += r.count;
-        double w = 0;
-        for (int g = 0; g < G; g++) {
-            jcost += (double)per[(size_t)g] * jump_cost_numbers(gens[(size_t)g].mexp, gens[(size_t)g].n);
-            w += (double)per[(size_t)g];
-            active += per[(size_t)g] > 0;
+        per[(size_t)r.gen] += r.count;
+        if (!seen[(size_t)r.gen]) {
+            seen[(size_t)r.gen] = 1;
+            named.push_back(r.gen);
         }
-        jcost = w > 0 ? jcost / w : 0;
     }
+    // Resident workgroups of the default configuration of the largest class
+    // among the plan's generators, and the numbers-weighted mean jump cost.
+    int slots = 0;
+    double jcost = 0, weight = 0;
+    int active = 0;
+    for (int g : named) {
+        const GenRef &gr = gens[(size_t)g];
+        const int b = gr.bits == 64, cls = c.class_of(gr.slot);
+        slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][c.layout[b][cls]]));
+        jcost += (double)per[(size_t)g] * jump_cost_numbers(gr.mexp, gr.n);
+        weight += (double)per[(size_t)g];
+        active += per[(size_t)g] > 0;
+    }
+    jcost = weight > 0 ? jcost / weight : 0;
     uint64_t S1 = 0, S2 = 0;
     double frac = 0;
     if (opt.cache)
