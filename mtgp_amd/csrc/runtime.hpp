@@ -77,7 +77,7 @@ struct Context {
     double jump_rounds_min = 0.35;
     // Two-phase launches of two-size grids (MTGP_OVERLAP=1).  Off by
     // default: on the batched MTGP32-11213 workload the side stream's jumps
-    // time-share the generation kernel's VALU rather than hide behind its
+    // share the generation kernel's power budget rather than hide behind its
     // stores (4.8 ms per step against 4.6 ms serial).
     bool overlap = false;
     // Batched jumps are split into partial jumps (chunk ranges, XORed when
@@ -209,10 +209,11 @@ struct PlanOptions {
 // phase 1.  launch() runs phase 0's jumps, then phase 0's generation on the
 // caller's stream while phase 1's jumps, then phase 1's generation, run on
 // a side stream; the caller's stream waits for the side stream at the end.
-// Phase 1's jumps (VALU work) so overlap generation (store-bound), and phase
-// 1's generation fills phase 0's tail.  launch_part runs the merged launches
-// (one jump and one generation launch per class over both phases: the
-// serial order).
+// Phase 1's jumps then run beside generation and phase 1's generation fills
+// phase 0's tail; measured on MI355X the jumps share the generation
+// kernel's power budget instead of hiding behind it, so the order is off by
+// default (set_overlap).  launch_part runs the merged launches (one jump
+// and one generation launch per class over both phases: the serial order).
 struct Plan {
     struct Launch {
         int bits, cls, layout, first, count, grid;
