@@ -74,3 +74,53 @@ class TensorGenSet(M.GenSet):
         s = stream if stream is not None else torch.cuda.current_stream()
         arr = M._requests(reqs)
         M._check(M._lib.mtgp_genset_generate(self.h, arr, len(reqs), C.c_void_p(s.cuda_stream)), "generate")
+
+
+class Generator:
+    """One MTGP generator whose stream fills torch tensors on the GPU, in
+    the manner of a torch.Generator.  Every fill is one plan: the stream is
+    cut into cells started by jump-ahead, so a single generator uses the
+    whole device, and consecutive fills continue the stream exactly as the
+    reference's genrand calls would.
+
+        g = mt.Generator(32, 11213, 0, seed=1234)
+        x = g.rand(1 << 20)                                  # float32, [0, 1)
+        g.random_(torch.empty(1 << 20, dtype=torch.int32, device="cuda"))
+        g.skip(1 << 40)                                      # by jump-ahead
+
+    MTGP32 generators fill 32-bit tensors (uint32 / int32 bits, float32),
+    MTGP64 ones 64-bit tensors (uint64 / int64 bits, float64)."""
+
+    def __init__(self, bits=32, mexp=11213, index=0, seed=0, array=None):
+        self.bits = bits
+        self._gs = TensorGenSet([(bits, mexp, index)])
+        if array is not None:
+            self._gs.init_array([[int(k) for k in array]])
+        else:
+            self._gs.init_seed([int(seed)])
+
+    def random_(self, tensor, stream=None):
+        """Raw output words (genrand_uint32 / genrand_uint64) into `tensor`."""
+        self._gs.generate_([(0, KIND_INT, tensor)], stream)
+        return tensor
+
+    def uniform_(self, tensor, kind=KIND_CLOSE_OPEN, stream=None):
+        """Floating-point outputs of `kind` (default [0, 1)) into `tensor`."""
+        self._gs.generate_([(0, kind, tensor)], stream)
+        return tensor
+
+    def rand(self, *size, kind=KIND_CLOSE_OPEN, device="cuda", stream=None):
+        """A new tensor of `size` (float32 for MTGP32, float64 for MTGP64)."""
+        dtype = torch.float32 if self.bits == 32 else torch.float64
+        return self.uniform_(torch.empty(*size, dtype=dtype, device=device), kind, stream)
+
+    def skip(self, n):
+        """Advances the stream by n numbers (stepped over or jumped)."""
+        self._gs.generate([(0, KIND_INT, 0, int(n), 0)])
+
+    def get_state(self):
+        """The state window (numpy, N words, oldest first)."""
+        return self._gs.state(0)
+
+    def set_state(self, window):
+        self._gs.set_state(0, window)

@@ -1,6 +1,7 @@
 """The PyTorch-ROCm front end (mtgp_amd/torch_ext.py), in its own process:
 torch imported first, one HIP runtime, numbers written in place into
-tensors on a torch stream, against the oracle."""
+tensors on a torch stream, and the torch.Generator-style object (fills,
+skip by jump-ahead, state), against the oracle."""
 import os
 import subprocess
 import sys
@@ -45,6 +46,30 @@ try:
     raise SystemExit("dtype mismatch not refused")
 except TypeError:
     pass
+# the torch.Generator-style object: one generator's stream across fills
+g = mt.Generator(32, 11213, 3, seed=99)
+p = o.params(32, 11213, 3)
+w = o.init_state(p, 99)
+x = g.rand(50000)
+e, w = o.generate(p, w, 2, 50000)
+assert x.dtype == torch.float32 and (x.cpu().numpy().view(np.uint32) == e.view(np.uint32)).all()
+r = g.random_(torch.empty(1000, dtype=torch.int32, device="cuda"))
+e, w = o.generate(p, w, 0, 1000)
+assert (r.cpu().numpy().view(np.uint32) == e).all()
+g.skip(1 << 40)
+import mtgp_amd as M
+P = o.hex_to_poly(M.charpoly_hex(32, 11213, 3), 11213 // 64 + 2)
+w = o.jump(p, w, o.poly_to_hex(o.xpow_mod(1 << 40, P)))
+y = g.rand(2, 500, kind=mt.KIND_OPEN_OPEN)
+e, w = o.generate(p, w, 4, 1000)
+assert tuple(y.shape) == (2, 500) and (y.cpu().numpy().reshape(-1).view(np.uint32) == e.view(np.uint32)).all()
+assert (g.get_state() == w).all()
+g64 = mt.Generator(64, 23209, 2, array=[1, 2, 3, 4])
+p64 = o.params(64, 23209, 2)
+w64 = o.init_by_array(p64, [1, 2, 3, 4])
+z = g64.rand(3000)
+e, w64 = o.generate(p64, w64, 2, 3000)
+assert z.dtype == torch.float64 and (z.cpu().numpy().view(np.uint64) == e.view(np.uint64)).all()
 print("TORCH-OK %.4f" % float(f.mean()))
 """
 
