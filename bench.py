@@ -203,7 +203,8 @@ def _time_plan(M, plan, stream, reps=3):
 def run_configs(M):
     """The other configurations, timed like the main one (start-state jump
     launch + generation launch per step, inputs resident): 1 (one stream),
-    3 (MTGP64 batches per MEXP) and 4 (4096 substreams)."""
+    3 (MTGP64 batches per MEXP) and 4 (4096 substreams), plus a batch of
+    MTGP64 dynamic-creator sets (the 128-word-step kernels)."""
     stream = M.Stream()
     out = {}
 
@@ -237,6 +238,21 @@ def run_configs(M):
         out["3-%d" % mexp] = line(G * 16 * n, *_time_plan(M, plan, stream), plan.stats()["items"],
                                   workload="MTGP64-%d sets 0..63: 2^24 uint64 then 2^24 double [0,1) each" % mexp)
         del plan, buf, gs
+    # MTGP64 dynamic-creator sets 0..63 (N = 176: the 128-word-step class),
+    # seeds 1..64, 2^24 uint64 then 2^24 double [0,1) each
+    G = 64
+    gs = M.GenSet([M.dc_params(64, i) for i in range(G)])
+    gs.init_seed(list(range(1, G + 1)))
+    buf = M.DeviceBuffer(G * 16 * n)
+    reqs = []
+    for i in range(G):
+        reqs.append((i, M.KIND_INT, n, 0, buf.address(16 * n * i)))
+        reqs.append((i, M.KIND_CLOSE_OPEN, n, 0, buf.address(16 * n * i + 8 * n)))
+    plan = gs.plan(reqs)
+    out["dc64"] = line(G * 16 * n, *_time_plan(M, plan, stream), plan.stats()["items"],
+                       workload="MTGP64-DC-11213 sets 0..63 (128-word steps): 2^24 uint64 then 2^24 double "
+                                "[0,1) each")
+    del plan, buf, gs
     # 4: 4096 substreams i * 3^162 of set 0, seed 777, 2^20 uint32 each
     S, count = 4096, 1 << 20
     base = M.GenSet([(32, 11213, 0)])
