@@ -203,8 +203,9 @@ def _time_plan(M, plan, stream, reps=3):
 def run_configs(M):
     """The other configurations, timed like the main one (start-state jump
     launch + generation launch per step, inputs resident): 1 (one stream),
-    3 (MTGP64 batches per MEXP) and 4 (4096 substreams), plus a batch of
-    MTGP64 dynamic-creator sets (the 128-word-step kernels)."""
+    3 (MTGP64 batches per MEXP) and 4 (4096 substreams), plus batches of the
+    larger-state MTGP32 classes and of MTGP64 dynamic-creator sets (the
+    128-word-step kernels)."""
     stream = M.Stream()
     out = {}
 
@@ -237,6 +238,21 @@ def run_configs(M):
         plan = gs.plan(reqs)
         out["3-%d" % mexp] = line(G * 16 * n, *_time_plan(M, plan, stream), plan.stats()["items"],
                                   workload="MTGP64-%d sets 0..63: 2^24 uint64 then 2^24 double [0,1) each" % mexp)
+        del plan, buf, gs
+    # The larger-state MTGP32 classes: sets 0..63 of 23209 and 44497, seeds
+    # 1..64, 2^24 uint32 then 2^24 float [0,1) each
+    for mexp in (23209, 44497):
+        G = 64
+        gs = M.GenSet([(32, mexp, i) for i in range(G)])
+        gs.init_seed(list(range(1, G + 1)))
+        buf = M.DeviceBuffer(G * 8 * n)
+        reqs = []
+        for i in range(G):
+            reqs.append((i, M.KIND_INT, n, 0, buf.address(8 * n * i)))
+            reqs.append((i, M.KIND_CLOSE_OPEN, n, 0, buf.address(8 * n * i + 4 * n)))
+        plan = gs.plan(reqs)
+        out["32-%d" % mexp] = line(G * 8 * n, *_time_plan(M, plan, stream), plan.stats()["items"],
+                                   workload="MTGP32-%d sets 0..63: 2^24 uint32 then 2^24 float [0,1) each" % mexp)
         del plan, buf, gs
     # MTGP64 dynamic-creator sets 0..63 (N = 176: the 128-word-step class),
     # seeds 1..64, 2^24 uint64 then 2^24 double [0,1) each

@@ -64,8 +64,9 @@ int mtgp_device_info(int *compute_units, size_t *hbm_bytes);
  * "small_rounds" (MTGP_SMALL_ROUNDS, small cells per slot of that grid),
  * "segment_rounds" (MTGP_SEGMENT_ROUNDS), "two_size_rho" (MTGP_TWO_SIZE_RHO,
  * largest jump cost / one-round cell size that keeps the two-size grid),
- * "jump_rounds_k" and "jump_rounds_min" (MTGP_JUMP_ROUNDS_K/_MIN: beyond
- * it, cells for k / rho rounds, at least min), "jump_waves_per_cu"
+ * "jump_floor_ms" (MTGP_JUMP_FLOOR_MS: beyond it, one round of 4, 8 or 12
+ * cells per CU chosen by a step-time model whose jump launch costs this
+ * much plus the jumps' own time), "jump_waves_per_cu"
  * (MTGP_JUMP_WAVES), "jump_max_parts" (MTGP_JUMP_MAX_PARTS, most partial
  * jumps a jump is split into), "overlap" (MTGP_OVERLAP, default two-phase
  * launch).

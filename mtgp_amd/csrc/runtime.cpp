@@ -122,10 +122,8 @@ Context &context() {
     }
     if (const char *e = getenv("MTGP_TWO_SIZE_RHO"))
         c->two_size_rho = atof(e);
-    if (const char *e = getenv("MTGP_JUMP_ROUNDS_MIN"))
-        c->jump_rounds_min = std::max(0.05, atof(e));
-    if (const char *e = getenv("MTGP_JUMP_ROUNDS_K"))
-        c->jump_rounds_k = std::max(0.01, atof(e));
+    if (const char *e = getenv("MTGP_JUMP_FLOOR_MS"))
+        c->jump_floor_ms = std::max(0.0, atof(e));
     if (const char *e = getenv("MTGP_JUMP_MAX_PARTS")) {
         const int v = atoi(e);
         if (v > 0 && v <= 64)
@@ -711,19 +709,52 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 double jump_cost_numbers(int mexp, int n) { return 0.026 * (double)mexp * (double)n; }
 
+GridModel grid_model_of(int bits, int cls, int mexp, int n) {
+    GridModel m;
+    m.jump_ns = 4.5e-5 * (double)mexp * (double)n * (double)(bits / 32) / 2;
+    m.w0 = bits == 64 ? 2.7 : 5.5;
+    m.rate = 4700;
+    // Step times of 64-set batches (2^24 + 2^24 numbers each) at 4, 8 and
+    // 12 cells per CU and of one MTGP32-11213 stream: jump launch time per
+    // jump and the generation rate's saturation in waves per CU.
+    if (bits == 32 && cls == 0)
+        m.jump_ns = 88, m.rate = 4900, m.w0 = 5.0;
+    else if (bits == 32 && cls == 1)
+        m.jump_ns = 375, m.rate = 3800;
+    else if (bits == 32 && cls == 2)
+        m.jump_ns = 880, m.rate = 3400;
+    else if (bits == 64 && cls == 0)
+        m.jump_ns = 380, m.rate = 4750;
+    else if (bits == 64 && cls == 1)
+        m.jump_ns = 1100, m.rate = 4450;
+    return m;
+}
+
 void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_gens, int slots, double jump_cost,
-               uint64_t &S1, uint64_t &S2, double &frac) {
+               const GridModel &model, uint64_t &S1, uint64_t &S2, double &frac) {
     const double usable = std::max(1.0, (double)slots - (double)n_requests);
     auto round256 = [](double v) { return std::max<uint64_t>(((uint64_t)v + 255) & ~255ull, 4096); };
     const double rho = jump_cost / std::max(1.0, (double)total_numbers / usable);
     if (rho > c.two_size_rho) {
-        // One round of cells on about r of the slots, as a whole number of
-        // cells per CU (an uneven count leaves some CUs one wave heavier,
-        // whose waves then finish last); each generator's last cell is on
-        // average half full.
-        const double r = std::min(1.0, std::max(c.jump_rounds_min, c.jump_rounds_k / rho));
-        const double per_cu = std::max(1.0, std::round(usable * r / (double)c.cus));
-        const double cells = std::max(1.0, per_cu * (double)c.cus - 0.5 * (double)n_gens);
+        // One round of equal cells, w per CU, w minimising the modelled step
+        // time (Context::two_size_rho).  w is a multiple of the 4 SIMDs of a
+        // CU: an uneven count leaves a CU, or a SIMD, one wave heavier, and
+        // its waves finish last (measured: MTGP64-44497 generates no faster
+        // at 5 per CU than at 4).
+        const int wmax = std::max(1, slots / std::max(1, c.cus));
+        double best = 0, best_t = 0;
+        for (int w = std::min(4, wmax); w <= wmax; w += 4) {
+            const double jumps = std::max(0.0, (double)w * c.cus - (double)n_gens);
+            const double t_jump = c.jump_floor_ms * 1e6 + jumps * model.jump_ns;
+            const double t_gen = model.bytes / (model.rate * (1.0 - std::exp(-(double)w / model.w0)));  // ns
+            const double t = t_jump + t_gen;
+            if (best == 0 || t < best_t) {
+                best = w;
+                best_t = t;
+            }
+        }
+        // each generator's last cell is on average half full
+        const double cells = std::max(1.0, best * (double)c.cus - 0.5 * (double)n_gens);
         frac = 0;
         S1 = S2 = round256(std::ceil((double)total_numbers / cells));
         return;
@@ -771,15 +802,31 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     int slots = 0;
     double jcost = 0, weight = 0;
     int active = 0;
+    GridModel model;
+    model.rate = 0;
+    double w0 = 0;
     for (int g : named) {
         const GenRef &gr = gens[(size_t)g];
         const int b = gr.bits == 64, cls = c.class_of(gr.slot);
         slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][c.layout[b][cls]]));
-        jcost += (double)per[(size_t)g] * jump_cost_numbers(gr.mexp, gr.n);
-        weight += (double)per[(size_t)g];
+        const double nw = (double)per[(size_t)g];
+        jcost += nw * jump_cost_numbers(gr.mexp, gr.n);
+        const GridModel gm = grid_model_of(gr.bits, cls, gr.mexp, gr.n);
+        model.jump_ns += nw * gm.jump_ns;
+        model.rate += nw * gm.rate;
+        model.bytes += nw * (gr.bits / 8);
+        w0 += nw * gm.w0;
+        weight += nw;
         active += per[(size_t)g] > 0;
     }
     jcost = weight > 0 ? jcost / weight : 0;
+    if (weight > 0) {
+        model.jump_ns /= weight;
+        model.rate /= weight;
+        model.w0 = w0 / weight;
+    } else {
+        model.rate = 4700;
+    }
     uint64_t S1 = 0, S2 = 0;
     double frac = 0;
     if (opt.cache)
@@ -787,7 +834,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     else if (opt.segment)
         S1 = S2 = opt.segment;
     else
-        auto_grid(c, all, nonempty, active, slots, jcost, S1, S2, frac);
+        auto_grid(c, all, nonempty, active, slots, jcost, model, S1, S2, frac);
     const uint64_t S = S1;
     // Requests as runs of stream positions per generator.
     struct Run {
@@ -1643,10 +1690,8 @@ double *tuning_knob(Context &c, const std::string &name) {
         return &c.small_rounds;
     if (name == "two_size_rho")
         return &c.two_size_rho;
-    if (name == "jump_rounds_k")
-        return &c.jump_rounds_k;
-    if (name == "jump_rounds_min")
-        return &c.jump_rounds_min;
+    if (name == "jump_floor_ms")
+        return &c.jump_floor_ms;
     return nullptr;
 }
 }  // namespace
@@ -1676,7 +1721,7 @@ int mtgp_tuning_set(const char *name, double value) {
     if (!k)
         throw Error("unknown tuning knob '" + n + "'");
     if (!(value >= 0) || (n == "segment_split" && value >= 1) || (n == "segment_rounds" && value < 0.25) ||
-        (n == "jump_rounds_min" && (value <= 0 || value > 1)) || (n == "jump_rounds_k" && value <= 0) ||
+        
         (n == "small_rounds" && (value < 0.1 || value > 8)))
         throw Error("value out of range for '" + n + "'");
     *k = value;

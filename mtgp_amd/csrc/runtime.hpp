@@ -65,16 +65,20 @@ struct Context {
     // Small cells of the two-size grid per resident slot (their count is
     // about small_rounds * slots; each costs a jump).
     double small_rounds = 1.0;
-    // Jump-cost-aware grid: with rho = (jump cost) / (cell size of one round
-    // over every slot), rho <= two_size_rho keeps the grid above; beyond it
-    // cells are sized for jump_rounds_k / rho rounds (at least
-    // jump_rounds_min, at most 1).  Measured on MI355X: MTGP64-23209
-    // (rho 0.41) best at 0.5 round, 3.66 TB/s against 2.60 TB/s with the
-    // two-size grid; MTGP64-44497 (rho 1.1) best at 0.35 round, 2.97 against
-    // 1.45 TB/s; MTGP32-11213 (rho 0.09) best with the two-size grid.
+    // Jump-cost-aware grid.  With rho = (jump cost) / (cell size of one
+    // round over every slot) <= two_size_rho the two-size grid above is
+    // used; beyond it, one round of equal cells, w per CU, with w chosen to
+    // minimise the modelled step time (w a multiple of 4, the SIMDs of a CU)
+    //   jump_floor_ms + (w * CUs - generators) * t_jump
+    //     + bytes / (rate * (1 - exp(-w / w0)))
+    // with the class's measured jump time, saturated generation rate and
+    // wave scale (grid_model_of; MTGP32 waves write half the bytes per step
+    // of MTGP64 ones and need more company).  It picks the measured best of
+    // 4, 8 and 12 cells per CU for every 64-set MTGP32/MTGP64 batch (e.g.
+    // MTGP32-23209: 1.66 TB/s at 4, 2.44 at 8, 2.52 at 12) and 4 for one
+    // MTGP32-11213 stream.
     double two_size_rho = 0.15;
-    double jump_rounds_k = 0.2;
-    double jump_rounds_min = 0.35;
+    double jump_floor_ms = 0.15;
     // Two-phase launches of two-size grids (MTGP_OVERLAP=1).  Off by
     // default: on the batched MTGP32-11213 workload the side stream's jumps
     // share the generation kernel's power budget rather than hide behind its
@@ -143,6 +147,18 @@ uint64_t auto_segment(const Context &c, uint64_t total_numbers, int n_requests, 
 // generate).  Measured on MI355X at about 0.026 * MEXP * N numbers for
 // MTGP32-11213 and MTGP64-23209 alike.
 double jump_cost_numbers(int mexp, int n);
+// The grid model's inputs (Context::two_size_rho): for one kernel class,
+// or numbers-weighted means over the generators of a plan.
+struct GridModel {
+    double bytes = 0;     // output bytes of the plan
+    double jump_ns = 0;   // chip time of one batched jump
+    double rate = 4700;   // saturated generation rate, GB/s
+    double w0 = 5.0;      // waves per CU at which the rate is 63% of it
+};
+// Measured on MI355X per (word size, state class); classes without a
+// measurement use jump_ns = 4.5e-5 * MEXP * N * (bits / 32) / 2 (the jump's
+// word XORs at the rate measured for MTGP32-11213 and MTGP64-23209).
+GridModel grid_model_of(int bits, int cls, int mexp, int n);
 // Cell sizes for automatic segmentation: S1 for the first `frac` of each
 // generator's positions, S2 after (frac = 0: one size).  `jump_cost` is the
 // numbers-weighted mean jump_cost_numbers of the plan's generators: when a
@@ -150,7 +166,7 @@ double jump_cost_numbers(int mexp, int n);
 // two-size grid (two rounds); otherwise one size, with fewer cells the
 // dearer a jump is.
 void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_gens, int slots, double jump_cost,
-               uint64_t &S1, uint64_t &S2, double &frac);
+               const GridModel &model, uint64_t &S1, uint64_t &S2, double &frac);
 
 // Jump polynomials x^pos mod P kept per parameter slot for positions on a
 // grid of `cell` numbers (pos = k * cell): table T1[k] = x^(k cell) for
