@@ -153,50 +153,53 @@ __device__ __forceinline__ void step32(uint32_t *buf, const uint32_t *tab, P32 p
 // Lane-strided variant of step32: lane l computes words l + 64i (i < 4),
 // so every LDS access of the step is lane-consecutive (no bank conflicts)
 // and every store instruction writes 256 contiguous bytes.  No alignment
-// requirement on b or dst.
-template <int KIND, bool OUT, bool FULL = false>
+// requirement on b or dst.  With W waves per workgroup the step is 256W
+// words (thread l computes l + 64Wi), which needs N - pos >= 256W.
+template <int KIND, bool OUT, bool FULL = false, int W = 1>
 __device__ __forceinline__ void step32s(uint32_t *buf, const uint32_t *tab, P32 p, int b,
                                         gu32 *dst, uint32_t m) {
+    constexpr int T = kWave * W;
     const int l = threadIdx.x;
     const uint32_t *xp = buf + b + l;
     const uint32_t *yp = xp + p.pm1;
     uint32_t r[4], t[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        t[i] = yp[kWave * i];
-        r[i] = rec32(p, tab, xp[kWave * i], xp[kWave * i + 1], yp[kWave * i + 1]);
+        t[i] = yp[T * i];
+        r[i] = rec32(p, tab, xp[T * i], xp[T * i + 1], yp[T * i + 1]);
     }
     uint32_t *wp = buf + b + p.n + l;
 #pragma unroll
     for (int i = 0; i < 4; i++)
-        wp[kWave * i] = r[i];
+        wp[T * i] = r[i];
     if (OUT) {
         uint32_t o[4];
 #pragma unroll
         for (int i = 0; i < 4; i++)
             o[i] = out32<KIND>(tab, r[i], t[i]);
         // the m leading outputs: lanes past them fall outside the window
-        const store_window_t w = store_window((uint64_t)(uintptr_t)dst, FULL ? 4u * kStep : 4u * m);
+        const store_window_t w = store_window((uint64_t)(uintptr_t)dst, FULL ? 4u * kStep * W : 4u * m);
 #pragma unroll
         for (int i = 0; i < 4; i++)
-            store_u32(w, 4u * (uint32_t)(l + kWave * i), o[i]);
+            store_u32(w, 4u * (uint32_t)(l + T * i), o[i]);
     }
 }
 
 // Copies the window [b, b + n) to [to, to + n) through registers (the
-// ranges may overlap).
-template <int NJ>
+// ranges may overlap); NJ words per thread of a W-wave workgroup.
+template <int NJ, int W = 1>
 __device__ __forceinline__ void relocate32(uint32_t *buf, int b, int n, int to = 0) {
+    constexpr int T = kWave * W;
     const int l = threadIdx.x;
     uint32_t v[NJ];
 #pragma unroll
     for (int m = 0; m < NJ; m++)
-        v[m] = (l + kWave * m < n) ? buf[b + l + kWave * m] : 0u;
+        v[m] = (l + T * m < n) ? buf[b + l + T * m] : 0u;
     wave_sync();
 #pragma unroll
     for (int m = 0; m < NJ; m++)
-        if (l + kWave * m < n)
-            buf[to + l + kWave * m] = v[m];
+        if (l + T * m < n)
+            buf[to + l + T * m] = v[m];
     wave_sync();
 }
 
@@ -285,36 +288,37 @@ __device__ __forceinline__ uint32_t head32(uint64_t out, uint64_t count) {
 
 // Window base for a segment with head h: layout 0 needs (b + h) % 4 == 0;
 // both need room for one more step.  Moves the window when either fails.
-template <int NJ, int NBUF, int LAYOUT>
+template <int NJ, int NBUF, int LAYOUT, int W = 1>
 __device__ __forceinline__ int place32(uint32_t *buf, P32 p, int b, uint32_t h) {
     const int to = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
     const bool misaligned = LAYOUT == 0 && (((uint32_t)b + h) & 3u) != 0;
-    if (misaligned || b + p.n + kStep > NBUF) {
-        relocate32<NJ>(buf, b, p.n, to);
+    if (misaligned || b + p.n + kStep * W > NBUF) {
+        relocate32<NJ, W>(buf, b, p.n, to);
         return to;
     }
     return b;
 }
 
 // Steps the window over `count` numbers without output; returns the new base.
-template <int NJ, int NBUF>
+template <int NJ, int NBUF, int W = 1>
 __device__ __forceinline__ int advance32(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t count) {
-    for (uint64_t s = count / kStep; s > 0; s--) {
-        if (b + p.n + kStep > NBUF) {
-            relocate32<NJ>(buf, b, p.n);
+    constexpr int S = kStep * W;
+    for (uint64_t s = count / S; s > 0; s--) {
+        if (b + p.n + S > NBUF) {
+            relocate32<NJ, W>(buf, b, p.n);
             b = 0;
         }
-        step32s<0, false>(buf, tab, p, b, nullptr, 0);
+        step32s<0, false, false, W>(buf, tab, p, b, nullptr, 0);
         wave_sync();
-        b += kStep;
+        b += S;
     }
-    const uint32_t m = (uint32_t)(count % kStep);
+    const uint32_t m = (uint32_t)(count % S);
     if (m) {
-        if (b + p.n + kStep > NBUF) {
-            relocate32<NJ>(buf, b, p.n);
+        if (b + p.n + S > NBUF) {
+            relocate32<NJ, W>(buf, b, p.n);
             b = 0;
         }
-        step32s<0, false>(buf, tab, p, b, nullptr, 0);
+        step32s<0, false, false, W>(buf, tab, p, b, nullptr, 0);
         wave_sync();
         b += (int)m;
     }
@@ -323,9 +327,11 @@ __device__ __forceinline__ int advance32(uint32_t *buf, const uint32_t *tab, P32
 
 // Writes `count` numbers of KIND to out from the window at base b (placed
 // by place32 for head h); returns the window base after the last number.
-template <int KIND, int NJ, int NBUF, int LAYOUT>
+template <int KIND, int NJ, int NBUF, int LAYOUT, int W = 1>
 __device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint32_t h,
                           gu32 *__restrict__ out, uint64_t count) {
+    static_assert(W == 1 || LAYOUT == 1, "multi-wave cells are lane-strided");
+    constexpr int S = kStep * W;
     const int l = threadIdx.x;
     gu32 *dst = out;
     uint64_t rest = count;
@@ -342,7 +348,7 @@ __device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P3
         } else {
             // h < 32 numbers so that every later 256-byte store starts on a
             // 128-byte line (no partially written lines).
-            step32s<KIND, true>(buf, tab, p, b, dst, h);
+            step32s<KIND, true, false, W>(buf, tab, p, b, dst, h);
         }
         wave_sync();
         b += (int)h;
@@ -351,29 +357,29 @@ __device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P3
     }
     // Whole steps, then one partial step.  (Written without a select on the
     // 64-bit remainder.)
-    for (uint64_t s = rest / kStep; s > 0; s--) {
-        if (b + p.n + kStep > NBUF) {
-            relocate32<NJ>(buf, b, p.n);
+    for (uint64_t s = rest / S; s > 0; s--) {
+        if (b + p.n + S > NBUF) {
+            relocate32<NJ, W>(buf, b, p.n);
             b = 0;
         }
         if (LAYOUT == 0)
             step32<KIND, true, true>(buf, tab, p, b, dst, kStep);
         else
-            step32s<KIND, true, true>(buf, tab, p, b, dst, kStep);
+            step32s<KIND, true, true, W>(buf, tab, p, b, dst, S);
         wave_sync();
-        b += kStep;
-        dst += kStep;
+        b += S;
+        dst += S;
     }
-    const uint32_t m = (uint32_t)(rest % kStep);
+    const uint32_t m = (uint32_t)(rest % S);
     if (m) {
-        if (b + p.n + kStep > NBUF) {
-            relocate32<NJ>(buf, b, p.n);
+        if (b + p.n + S > NBUF) {
+            relocate32<NJ, W>(buf, b, p.n);
             b = 0;
         }
         if (LAYOUT == 0)
             step32<KIND, true>(buf, tab, p, b, dst, m);
         else
-            step32s<KIND, true>(buf, tab, p, b, dst, m);
+            step32s<KIND, true, false, W>(buf, tab, p, b, dst, m);
         wave_sync();
         b += (int)m;
     }
@@ -381,50 +387,55 @@ __device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P3
 }
 
 // One segment (Segment, common.hpp) from window base b; returns the new base.
-template <int NJ, int NBUF, int LAYOUT>
+template <int NJ, int NBUF, int LAYOUT, int W = 1>
 __device__ __forceinline__ int segment32_body(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t out,
                                               uint64_t count, int kind) {
     if (count == 0)
         return b;
     if (out == 0)
-        return advance32<NJ, NBUF>(buf, tab, p, b, count);
+        return advance32<NJ, NBUF, W>(buf, tab, p, b, count);
     const uint32_t h = head32<LAYOUT>(out, count);
-    b = place32<NJ, NBUF, LAYOUT>(buf, p, b, h);
+    b = place32<NJ, NBUF, LAYOUT, W>(buf, p, b, h);
     gu32 *dst = reinterpret_cast<gu32 *>(out);
     switch (kind) {
-    case 0: return generate32<0, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
-    case 1: return generate32<1, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
-    case 2: return generate32<2, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
-    case 3: return generate32<3, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
-    default: return generate32<4, NJ, NBUF, LAYOUT>(buf, tab, p, b, h, dst, count);
+    case 0: return generate32<0, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
+    case 1: return generate32<1, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
+    case 2: return generate32<2, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
+    case 3: return generate32<3, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
+    default: return generate32<4, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
     }
 }
 
 // The small state classes (NJ <= 6) inline the segment into the kernel, so
 // that LDS offsets fold into the instructions; the large ones call an
 // out-of-line copy (fully inlined, their kernels spill registers).
-template <int NJ, int NBUF, int LAYOUT>
+template <int NJ, int NBUF, int LAYOUT, int W>
 __device__ __noinline__ int segment32_call(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t out,
                                            uint64_t count, int kind) {
-    return segment32_body<NJ, NBUF, LAYOUT>(buf, tab, p, b, out, count, kind);
+    return segment32_body<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, out, count, kind);
 }
-template <int NJ, int NBUF, int LAYOUT>
+template <int NJ, int NBUF, int LAYOUT, int W = 1>
 __device__ __forceinline__ int segment32(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t out,
                                          uint64_t count, int kind) {
     if constexpr (NJ <= 6)
-        return segment32_body<NJ, NBUF, LAYOUT>(buf, tab, p, b, out, count, kind);
+        return segment32_body<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, out, count, kind);
     else
-        return segment32_call<NJ, NBUF, LAYOUT>(buf, tab, p, b, out, count, kind);
+        return segment32_call<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, out, count, kind);
 }
 
-template <int NMAX, int K, int LAYOUT>
-__global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict__ items, int n_items,
-                                                          const DevParam *__restrict__ params,
-                                                          uint32_t *__restrict__ queue) {
-    constexpr int NJ = (NMAX + kWave - 1) / kWave;
-    constexpr int NBUF = K * kStep + kWave * NJ + kWave;
+// W waves per workgroup (W > 1: lane-strided only, every set of the
+// class has N - pos >= 256W, and no Item carries a fused jump polynomial;
+// the planner guarantees all three).
+template <int NMAX, int K, int LAYOUT, int W = 1>
+__global__ __launch_bounds__(64 * W) void mtgp32_items_kernel(const Item *__restrict__ items, int n_items,
+                                                              const DevParam *__restrict__ params,
+                                                              uint32_t *__restrict__ queue) {
+    constexpr int T = kWave * W;
+    constexpr int NJ = (NMAX + T - 1) / T;
+    constexpr int NBUF = K * kStep * W + T * NJ + T;
     __shared__ __attribute__((aligned(16))) uint32_t buf[NBUF];
     __shared__ uint32_t tab[kTabWords];
+    __shared__ uint32_t next_item;
     const int l = threadIdx.x;
     uint32_t it = blockIdx.x;
     while (it < (uint32_t)n_items) {
@@ -441,18 +452,23 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
         gu32 *state_out = reinterpret_cast<gu32 *>(I.state_out);
         const gu32 *state_in = reinterpret_cast<const gu32 *>(I.state_in);
         wave_sync();
-        if (I.poly) {
-            uint32_t acc[NJ];
-            jump32<NJ, K, NBUF>(buf, tab, p, state_in, reinterpret_cast<const gu32 *>(I.poly),
-                                I.poly_words, acc);
+        bool fused = false;
+        if constexpr (W == 1) {
+            if (I.poly) {
+                fused = true;
+                uint32_t acc[NJ];
+                jump32<NJ, K, NBUF>(buf, tab, p, state_in, reinterpret_cast<const gu32 *>(I.poly),
+                                    I.poly_words, acc);
 #pragma unroll
-            for (int m = 0; m < NJ; m++)
-                if (l + kWave * m < p.n)
-                    buf[l + kWave * m] = acc[m];
-        } else {
+                for (int m = 0; m < NJ; m++)
+                    if (l + kWave * m < p.n)
+                        buf[l + kWave * m] = acc[m];
+            }
+        }
+        if (!fused) {
             const int parts = I.parts > 1 ? I.parts : 1;
             const uint64_t pstride = I.part_bytes / 4;
-            for (int j = l; j < p.n; j += kWave) {
+            for (int j = l; j < p.n; j += T) {
                 uint32_t v = state_in[j];
                 for (int q = 1; q < parts; q++)
                     v ^= state_in[(uint64_t)q * pstride + (uint64_t)j];
@@ -464,18 +480,27 @@ __global__ __launch_bounds__(64) void mtgp32_items_kernel(const Item *__restrict
         if (I.nsegs > 0) {
             const GMEM Segment *segs = reinterpret_cast<const GMEM Segment *>(I.segs);
             for (int si = 0; si < I.nsegs; si++)
-                b = segment32<NJ, NBUF, LAYOUT>(buf, tab, p, b, segs[si].out, segs[si].count, segs[si].kind);
+                b = segment32<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, segs[si].out, segs[si].count, segs[si].kind);
         } else {
-            b = segment32<NJ, NBUF, LAYOUT>(buf, tab, p, b, I.out, I.count, I.kind);
+            b = segment32<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, I.out, I.count, I.kind);
         }
         if (state_out)
-            for (int j = l; j < p.n; j += kWave)
+            for (int j = l; j < p.n; j += T)
                 state_out[j] = buf[b + j];
         wave_sync();
-        uint32_t next = 0;
-        if (l == 0)
-            next = atomicAdd(&queue[0], 1u);
-        it = gridDim.x + uniform(__shfl(next, 0));
+        if constexpr (W == 1) {
+            uint32_t next = 0;
+            if (l == 0)
+                next = atomicAdd(&queue[0], 1u);
+            it = gridDim.x + uniform(__shfl(next, 0));
+        } else {
+            // One claim per workgroup, handed to every wave through LDS.
+            if (l == 0)
+                next_item = atomicAdd(&queue[0], 1u);
+            wave_sync();
+            it = gridDim.x + uniform(next_item);
+            wave_sync();
+        }
     }
     if (l == 0) {
         __threadfence();
@@ -586,25 +611,27 @@ __device__ __forceinline__ void step64(uint32_t *bh, uint32_t *bl, const uint32_
 }
 
 // Lane-strided variant of step64 (see step32s); outputs are 8-byte stores,
-// 512 contiguous bytes per instruction.
-template <int KIND, bool OUT, bool FULL = false, int WPL = 4>
+// 512 contiguous bytes per instruction.  With WV waves per workgroup the
+// step is 64 * WV * WPL words (thread l computes l + 64WVi).
+template <int KIND, bool OUT, bool FULL = false, int WPL = 4, int WV = 1>
 __device__ __forceinline__ void step64s(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b,
                                         gu32 *dst, uint32_t m) {
+    constexpr int T = kWave * WV;
     const int l = threadIdx.x;
     const int q = b + l;
     const int yq = q + p.pos - 1;
     uint32_t rh[WPL], rl[WPL], tl[WPL];
 #pragma unroll
     for (int i = 0; i < WPL; i++) {
-        const int x = q + kWave * i, y = yq + kWave * i;
+        const int x = q + T * i, y = yq + T * i;
         tl[i] = bl[y];
         rec64(p, tab, bh[x], bl[x], bh[x + 1], bl[x + 1], bh[y + 1], bl[y + 1], rh[i], rl[i]);
     }
     const int w = q + p.n;
 #pragma unroll
     for (int i = 0; i < WPL; i++) {
-        bh[w + kWave * i] = rh[i];
-        bl[w + kWave * i] = rl[i];
+        bh[w + T * i] = rh[i];
+        bl[w + T * i] = rl[i];
     }
     if (OUT) {
         gu2 *d2 = reinterpret_cast<gu2 *>(dst);
@@ -614,29 +641,30 @@ __device__ __forceinline__ void step64s(uint32_t *bh, uint32_t *bl, const uint32
             out64<KIND>(tab, rh[i], rl[i], tl[i], oh[i], ol[i]);
 #pragma unroll
         for (int i = 0; i < WPL; i++) {
-            const uint32_t e = (uint32_t)(l + kWave * i);
+            const uint32_t e = (uint32_t)(l + T * i);
             if (FULL || e < m)
                 d2[e] = v2u{ol[i], oh[i]};
         }
     }
 }
 
-template <int NJ>
+template <int NJ, int WV = 1>
 __device__ __forceinline__ void relocate64(uint32_t *bh, uint32_t *bl, int b, int n, int to = 0) {
+    constexpr int T = kWave * WV;
     const int l = threadIdx.x;
     uint32_t vh[NJ], vl[NJ];
 #pragma unroll
     for (int m = 0; m < NJ; m++) {
-        const bool ok = l + kWave * m < n;
-        vh[m] = ok ? bh[b + l + kWave * m] : 0u;
-        vl[m] = ok ? bl[b + l + kWave * m] : 0u;
+        const bool ok = l + T * m < n;
+        vh[m] = ok ? bh[b + l + T * m] : 0u;
+        vl[m] = ok ? bl[b + l + T * m] : 0u;
     }
     wave_sync();
 #pragma unroll
     for (int m = 0; m < NJ; m++)
-        if (l + kWave * m < n) {
-            bh[to + l + kWave * m] = vh[m];
-            bl[to + l + kWave * m] = vl[m];
+        if (l + T * m < n) {
+            bh[to + l + T * m] = vh[m];
+            bl[to + l + T * m] = vl[m];
         }
     wave_sync();
 }
@@ -723,47 +751,47 @@ __device__ __forceinline__ uint32_t head64(uint64_t out, uint64_t count) {
 // WPL (words per lane and step): 4 = 256-word steps; 2 = 128-word steps for
 // states with N - pos < 256 (the MTGP64 dynamic-creator sets, N = 176),
 // lane-strided layout only.
-template <int NJ, int NBUF, int LAYOUT, int WPL = 4>
+template <int NJ, int NBUF, int LAYOUT, int WPL = 4, int WV = 1>
 __device__ __forceinline__ int place64(uint32_t *bh, uint32_t *bl, P64 p, int b, uint32_t h) {
     const int to = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
     const bool misaligned = LAYOUT == 0 && (((uint32_t)b + h) & 3u) != 0;
-    if (misaligned || b + p.n + kWave * WPL > NBUF) {
-        relocate64<NJ>(bh, bl, b, p.n, to);
+    if (misaligned || b + p.n + kWave * WV * WPL > NBUF) {
+        relocate64<NJ, WV>(bh, bl, b, p.n, to);
         return to;
     }
     return b;
 }
 
-template <int NJ, int NBUF, int WPL = 4>
+template <int NJ, int NBUF, int WPL = 4, int WV = 1>
 __device__ __forceinline__ int advance64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b, uint64_t count) {
-    constexpr int W = kWave * WPL;
+    constexpr int W = kWave * WV * WPL;
     for (uint64_t s = count / W; s > 0; s--) {
         if (b + p.n + W > NBUF) {
-            relocate64<NJ>(bh, bl, b, p.n);
+            relocate64<NJ, WV>(bh, bl, b, p.n);
             b = 0;
         }
-        step64s<0, false, false, WPL>(bh, bl, tab, p, b, nullptr, 0);
+        step64s<0, false, false, WPL, WV>(bh, bl, tab, p, b, nullptr, 0);
         wave_sync();
         b += W;
     }
     const uint32_t m = (uint32_t)(count % W);
     if (m) {
         if (b + p.n + W > NBUF) {
-            relocate64<NJ>(bh, bl, b, p.n);
+            relocate64<NJ, WV>(bh, bl, b, p.n);
             b = 0;
         }
-        step64s<0, false, false, WPL>(bh, bl, tab, p, b, nullptr, 0);
+        step64s<0, false, false, WPL, WV>(bh, bl, tab, p, b, nullptr, 0);
         wave_sync();
         b += (int)m;
     }
     return b;
 }
 
-template <int KIND, int NJ, int NBUF, int LAYOUT, int WPL = 4>
+template <int KIND, int NJ, int NBUF, int LAYOUT, int WPL = 4, int WV = 1>
 __device__ __forceinline__ int generate64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b, uint32_t h,
                           gu32 *__restrict__ out, uint64_t count) {
-    static_assert(LAYOUT == 1 || WPL == 4, "128-word steps are lane-strided");
-    constexpr int W = kWave * WPL;
+    static_assert(LAYOUT == 1 || (WPL == 4 && WV == 1), "128-word steps and multi-wave cells are lane-strided");
+    constexpr int W = kWave * WV * WPL;
     const int l = threadIdx.x;
     gu32 *dst = out;
     uint64_t rest = count;
@@ -779,7 +807,7 @@ __device__ __forceinline__ int generate64(uint32_t *bh, uint32_t *bl, const uint
                 reinterpret_cast<gu2 *>(out)[l] = v2u{ol, oh};
             }
         } else {
-            step64s<KIND, true, false, WPL>(bh, bl, tab, p, b, dst, h);
+            step64s<KIND, true, false, WPL, WV>(bh, bl, tab, p, b, dst, h);
         }
         wave_sync();
         b += (int)h;
@@ -788,13 +816,13 @@ __device__ __forceinline__ int generate64(uint32_t *bh, uint32_t *bl, const uint
     }
     for (uint64_t s = rest / W; s > 0; s--) {
         if (b + p.n + W > NBUF) {
-            relocate64<NJ>(bh, bl, b, p.n);
+            relocate64<NJ, WV>(bh, bl, b, p.n);
             b = 0;
         }
         if (LAYOUT == 0)
             step64<KIND, true, true>(bh, bl, tab, p, b, dst, kStep);
         else
-            step64s<KIND, true, true, WPL>(bh, bl, tab, p, b, dst, W);
+            step64s<KIND, true, true, WPL, WV>(bh, bl, tab, p, b, dst, W);
         wave_sync();
         b += W;
         dst += 2 * W;
@@ -802,64 +830,67 @@ __device__ __forceinline__ int generate64(uint32_t *bh, uint32_t *bl, const uint
     const uint32_t m = (uint32_t)(rest % W);
     if (m) {
         if (b + p.n + W > NBUF) {
-            relocate64<NJ>(bh, bl, b, p.n);
+            relocate64<NJ, WV>(bh, bl, b, p.n);
             b = 0;
         }
         if (LAYOUT == 0)
             step64<KIND, true>(bh, bl, tab, p, b, dst, m);
         else
-            step64s<KIND, true, false, WPL>(bh, bl, tab, p, b, dst, m);
+            step64s<KIND, true, false, WPL, WV>(bh, bl, tab, p, b, dst, m);
         wave_sync();
         b += (int)m;
     }
     return b;
 }
 
-template <int NJ, int NBUF, int LAYOUT, int WPL>
+template <int NJ, int NBUF, int LAYOUT, int WPL, int WV = 1>
 __device__ __forceinline__ int segment64_body(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b,
                                               uint64_t out, uint64_t count, int kind) {
     if (count == 0)
         return b;
     if (out == 0)
-        return advance64<NJ, NBUF, WPL>(bh, bl, tab, p, b, count);
+        return advance64<NJ, NBUF, WPL, WV>(bh, bl, tab, p, b, count);
     const uint32_t h = head64<LAYOUT>(out, count);
-    b = place64<NJ, NBUF, LAYOUT, WPL>(bh, bl, p, b, h);
+    b = place64<NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, p, b, h);
     gu32 *dst = reinterpret_cast<gu32 *>(out);
     switch (kind) {
-    case 0: return generate64<0, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
-    case 1: return generate64<1, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
-    case 2: return generate64<2, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
-    case 3: return generate64<3, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
-    default: return generate64<4, NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, h, dst, count);
+    case 0: return generate64<0, NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, h, dst, count);
+    case 1: return generate64<1, NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, h, dst, count);
+    case 2: return generate64<2, NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, h, dst, count);
+    case 3: return generate64<3, NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, h, dst, count);
+    default: return generate64<4, NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, h, dst, count);
     }
 }
 
 // Inlined for the 2-words-per-lane class, an out-of-line call otherwise:
 // inlined, the 4-words-per-lane kernels need about 180 VGPRs (2 waves per
 // SIMD) against about 100 with the call.
-template <int NJ, int NBUF, int LAYOUT, int WPL>
+template <int NJ, int NBUF, int LAYOUT, int WPL, int WV>
 __device__ __noinline__ int segment64_call(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b,
                                            uint64_t out, uint64_t count, int kind) {
-    return segment64_body<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, out, count, kind);
+    return segment64_body<NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, out, count, kind);
 }
-template <int NJ, int NBUF, int LAYOUT, int WPL = 4>
+template <int NJ, int NBUF, int LAYOUT, int WPL = 4, int WV = 1>
 __device__ __forceinline__ int segment64(uint32_t *bh, uint32_t *bl, const uint32_t *tab, P64 p, int b,
                                          uint64_t out, uint64_t count, int kind) {
     if constexpr (WPL == 2)
-        return segment64_body<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, out, count, kind);
+        return segment64_body<NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, out, count, kind);
     else
-        return segment64_call<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, out, count, kind);
+        return segment64_call<NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, out, count, kind);
 }
 
-template <int NMAX, int K, int LAYOUT, int WPL = 4>
-__global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict__ items, int n_items,
-                                                          const DevParam *__restrict__ params,
-                                                          uint32_t *__restrict__ queue) {
-    constexpr int NJ = (NMAX + kWave - 1) / kWave;
-    constexpr int NBUF = K * kWave * WPL + kWave * NJ + kWave;
+// WV waves per workgroup: as for mtgp32_items_kernel.
+template <int NMAX, int K, int LAYOUT, int WPL = 4, int WV = 1>
+__global__ __launch_bounds__(64 * WV) void mtgp64_items_kernel(const Item *__restrict__ items, int n_items,
+                                                               const DevParam *__restrict__ params,
+                                                               uint32_t *__restrict__ queue) {
+    constexpr int T = kWave * WV;
+    constexpr int NJ = (NMAX + T - 1) / T;
+    constexpr int NBUF = K * T * WPL + T * NJ + T;
     __shared__ __attribute__((aligned(16))) uint32_t bh[NBUF];
     __shared__ __attribute__((aligned(16))) uint32_t bl[NBUF];
     __shared__ uint32_t tab[kTabWords];
+    __shared__ uint32_t next_item;
     const int l = threadIdx.x;
     uint32_t it = blockIdx.x;
     while (it < (uint32_t)n_items) {
@@ -876,9 +907,10 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         gu64 *state_out = reinterpret_cast<gu64 *>(I.state_out);
         const gu64 *state_in = reinterpret_cast<const gu64 *>(I.state_in);
         wave_sync();
-        // (128-word steps: the host gives such items batched start states, so
-        // the fused jump, which steps 256 words at a time, is compiled out.)
-        if (WPL == 4 && I.poly) {
+        // (128-word steps and multi-wave cells: the host gives such items
+        // batched start states, so the fused jump, which steps 256 words at a
+        // time on one wave, is compiled out.)
+        if (WPL == 4 && WV == 1 && I.poly) {
             uint32_t ah[NJ], al[NJ];
             jump64<NJ, K, NBUF>(bh, bl, tab, p, state_in, reinterpret_cast<const gu32 *>(I.poly),
                                 I.poly_words, ah, al);
@@ -891,7 +923,7 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         } else {
             const int parts = I.parts > 1 ? I.parts : 1;
             const uint64_t pstride = I.part_bytes / 8;
-            for (int j = l; j < p.n; j += kWave) {
+            for (int j = l; j < p.n; j += T) {
                 uint64_t v = state_in[j];
                 for (int q = 1; q < parts; q++)
                     v ^= state_in[(uint64_t)q * pstride + (uint64_t)j];
@@ -904,19 +936,28 @@ __global__ __launch_bounds__(64) void mtgp64_items_kernel(const Item *__restrict
         if (I.nsegs > 0) {
             const GMEM Segment *segs = reinterpret_cast<const GMEM Segment *>(I.segs);
             for (int si = 0; si < I.nsegs; si++)
-                b = segment64<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, segs[si].out, segs[si].count,
-                                                     segs[si].kind);
+                b = segment64<NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, segs[si].out, segs[si].count,
+                                                         segs[si].kind);
         } else {
-            b = segment64<NJ, NBUF, LAYOUT, WPL>(bh, bl, tab, p, b, I.out, I.count, I.kind);
+            b = segment64<NJ, NBUF, LAYOUT, WPL, WV>(bh, bl, tab, p, b, I.out, I.count, I.kind);
         }
         if (state_out)
-            for (int j = l; j < p.n; j += kWave)
+            for (int j = l; j < p.n; j += T)
                 state_out[j] = ((uint64_t)bh[b + j] << 32) | bl[b + j];
         wave_sync();
-        uint32_t next = 0;
-        if (l == 0)
-            next = atomicAdd(&queue[0], 1u);
-        it = gridDim.x + uniform(__shfl(next, 0));
+        if constexpr (WV == 1) {
+            uint32_t next = 0;
+            if (l == 0)
+                next = atomicAdd(&queue[0], 1u);
+            it = gridDim.x + uniform(__shfl(next, 0));
+        } else {
+            // One claim per workgroup, handed to every wave through LDS.
+            if (l == 0)
+                next_item = atomicAdd(&queue[0], 1u);
+            wave_sync();
+            it = gridDim.x + uniform(next_item);
+            wave_sync();
+        }
     }
     if (l == 0) {
         __threadfence();
@@ -1161,19 +1202,22 @@ constexpr int kNarrowN = 176;
 
 template <int B, int NMAX, int LAYOUT, int WPL = 4>
 struct KernelOf;
+// MTGP32 layout 2: lane-strided two-wave cells (512-word steps).
 template <int NMAX, int LAYOUT>
 struct KernelOf<32, NMAX, LAYOUT, 4> {
-    static constexpr auto fn = mtgp32_items_kernel<NMAX, kK, LAYOUT>;
+    static constexpr int waves = LAYOUT == 2 ? 2 : 1;
+    static constexpr auto fn = mtgp32_items_kernel<NMAX, kK, LAYOUT == 2 ? 1 : LAYOUT, waves>;
 };
 template <int NMAX, int LAYOUT, int WPL>
 struct KernelOf<64, NMAX, LAYOUT, WPL> {
-    static constexpr auto fn = mtgp64_items_kernel<NMAX, kK, LAYOUT, WPL>;
+    static constexpr int waves = LAYOUT == 2 ? 2 : 1;
+    static constexpr auto fn = mtgp64_items_kernel<NMAX, kK, LAYOUT == 2 ? 1 : LAYOUT, WPL, waves>;
 };
 
 template <class KO>
 hipError_t launch_one(const Item *items, int n_items, const DevParam *params, uint32_t *queue, int grid,
                       size_t dyn_lds, hipStream_t s) {
-    hipLaunchKernelGGL(KO::fn, dim3(grid), dim3(kWave), dyn_lds, s, items, n_items, params, queue);
+    hipLaunchKernelGGL(KO::fn, dim3(grid), dim3(kWave * KO::waves), dyn_lds, s, items, n_items, params, queue);
     return hipGetLastError();
 }
 
@@ -1181,14 +1225,18 @@ hipError_t launch_one(const Item *items, int n_items, const DevParam *params, ui
 // Class 3 (64-bit only) is the narrow class: 128-word steps, layout 1.
 template <class F>
 auto with_kernel(int bits, int cls, int layout, F &&f) {
+    if (!layout_supported(bits, cls, layout))
+        layout = 1;
     if (bits == 64 && cls == 3)
         return f.template operator()<KernelOf<64, kNarrowN, 1, 2>>();
 #define MTGP_CASE(B, N, L) \
     if (bits == B && N_OF(B, cls) == N && layout == L) return f.template operator()<KernelOf<B, N, L>>();
 #define N_OF(B, c) ((B) == 32 ? ((c) == 0 ? 351 : (c) == 1 ? 726 : 1391) : ((c) == 0 ? 363 : (c) == 1 ? 696 : 1727))
     MTGP_CASE(32, 351, 0) MTGP_CASE(32, 351, 1) MTGP_CASE(32, 726, 0) MTGP_CASE(32, 726, 1)
-    MTGP_CASE(32, 1391, 0) MTGP_CASE(32, 1391, 1) MTGP_CASE(64, 363, 0) MTGP_CASE(64, 363, 1)
+    MTGP_CASE(32, 1391, 0) MTGP_CASE(32, 1391, 1) MTGP_CASE(32, 726, 2) MTGP_CASE(32, 1391, 2)
+    MTGP_CASE(64, 363, 0) MTGP_CASE(64, 363, 1)
     MTGP_CASE(64, 696, 0) MTGP_CASE(64, 696, 1) MTGP_CASE(64, 1727, 0) MTGP_CASE(64, 1727, 1)
+    MTGP_CASE(64, 696, 2) MTGP_CASE(64, 1727, 2)
 #undef MTGP_CASE
 #undef N_OF
     return f.template operator()<KernelOf<64, 1727, 1>>();
@@ -1213,6 +1261,20 @@ int kernel_class(int bits, int n, int pos) {
 
 const void *kernel_fn(int bits, int cls, int layout) {
     return with_kernel(bits, cls, layout, []<class KO>() { return reinterpret_cast<const void *>(KO::fn); });
+}
+
+int kernel_threads(int bits, int cls, int layout) {
+    return with_kernel(bits, cls, layout, []<class KO>() { return kWave * KO::waves; });
+}
+
+bool layout_supported(int bits, int cls, int layout) {
+    if (layout < 0 || layout >= kLayouts)
+        return false;
+    if (bits == 64 && cls == 3)
+        return layout == 1;
+    if (layout == 2)
+        return cls == 1 || cls == 2;
+    return true;
 }
 
 hipError_t launch_items(int bits, int cls, int layout, const Item *items, int n_items, const DevParam *params,

@@ -15,10 +15,15 @@ int kernel_class(int bits, int n, int pos);
 constexpr int kClasses = 4;  // classes 0..2 of both word sizes, 3 for 64-bit
 // Step layouts: 0 = each lane computes 4 consecutive words and stores them
 // with one 128-bit store; 1 = lane l computes words l + 64i (conflict-free
-// LDS, 256-byte coalesced dword stores).
-constexpr int kLayouts = 2;
-// Kernel entry (for occupancy queries).
+// LDS, 256-byte coalesced dword stores); 2 = layout 1 with two waves per
+// cell, thread l computing words l + 128i of 512-word steps (classes 1 and
+// 2 of both word sizes; needs N - pos >= 512 and batched start states).
+constexpr int kLayouts = 3;
+// Whether (bits, cls) has a kernel of that layout.
+bool layout_supported(int bits, int cls, int layout);
+// Kernel entry (for occupancy queries) and its workgroup size.
 const void *kernel_fn(int bits, int cls, int layout);
+int kernel_threads(int bits, int cls, int layout);
 // Runs items[0, n_items) (device array, all of word size `bits` and class
 // <= cls) with `grid` persistent workgroups.  `queue` is a device pair of
 // uint32_t counters that is zero on entry and zero again on exit.

@@ -86,8 +86,11 @@ Context &context() {
             for (int lay = 0; lay < kLayouts; lay++) {
                 if (b == 0 && cls == 3)
                     continue;
+                // (a layout the class lacks runs as layout 1)
+                const int kl = layout_supported(b ? 64 : 32, cls, lay) ? lay : 1;
                 int nb = 0;
-                hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_fn(b ? 64 : 32, cls, lay), 64, 0),
+                hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_fn(b ? 64 : 32, cls, kl),
+                                                                       kernel_threads(b ? 64 : 32, cls, kl), 0),
                           "hipOccupancyMaxActiveBlocksPerMultiprocessor");
                 c->blocks_per_cu[b][cls][lay] = nb > 0 ? nb : 1;
             }
@@ -144,7 +147,8 @@ Context &context() {
         if (lay >= 0 && lay < kLayouts)
             for (int b = 0; b < 2; b++)
                 for (int cls = 0; cls < 3; cls++)  // (class 3 is lane-strided only)
-                    c->layout[b][cls] = lay;
+                    if (layout_supported(b ? 64 : 32, cls, lay))
+                        c->layout[b][cls] = lay;
     }
     Context &ref = *c;
     g_ctx[dev] = std::move(c);
@@ -156,7 +160,7 @@ void select_device(int dev) {
     g_device = dev;
 }
 
-size_t Context::lds_reserve(const void *fn, int per_cu) {
+size_t Context::lds_reserve(const void *fn, int threads, int per_cu) {
     // Off unless asked for: measured neutral at 16 per CU.
     const char *e = getenv("MTGP_LDS_CAP");
     if (!e || atoi(e) == 0)
@@ -167,7 +171,8 @@ size_t Context::lds_reserve(const void *fn, int per_cu) {
     if (it != reserve_cache.end())
         return it->second;
     int base = 0;
-    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, fn, 64, 0), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, fn, threads, 0),
+              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     size_t dyn = 0;
     if (base > per_cu) {
         hipFuncAttributes attr;
@@ -222,6 +227,16 @@ int Context::slot_of(const DevParam &rec, const unsigned char *sha1) {
 int Context::class_of(int slot) const {
     const DevParam &d = param(slot);
     return kernel_class(d.bits, d.n, d.pos);
+}
+
+bool Context::wide_ok(int slot) const {
+    const DevParam &d = param(slot);
+    return d.n - d.pos >= 512 && layout_supported(d.bits, kernel_class(d.bits, d.n, d.pos), 2);
+}
+
+int Context::layout_for(int bits, int cls, bool wide) const {
+    const int lay = layout[bits == 64][cls];
+    return lay == 2 && !(wide && layout_supported(bits, cls, 2)) ? 1 : lay;
 }
 
 const DevParam &Context::param(int slot) const {
@@ -746,7 +761,8 @@ void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_g
         for (int w = std::min(4, wmax); w <= wmax; w += 4) {
             const double jumps = std::max(0.0, (double)w * c.cus - (double)n_gens);
             const double t_jump = c.jump_floor_ms * 1e6 + jumps * model.jump_ns;
-            const double t_gen = model.bytes / (model.rate * (1.0 - std::exp(-(double)w / model.w0)));  // ns
+            const double waves = (double)w * model.cell_waves;
+            const double t_gen = model.bytes / (model.rate * (1.0 - std::exp(-waves / model.w0)));  // ns
             const double t = t_jump + t_gen;
             if (best == 0 || t < best_t) {
                 best = w;
@@ -797,6 +813,25 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             named.push_back(r.gen);
         }
     }
+    // Start states: batched jumps unless fused ones are asked for;
+    // narrow-class items always need batched start states.
+    jump_mode = 1;
+    if (opt.jump_mode >= 0)
+        jump_mode = opt.jump_mode ? 1 : 0;
+    else if (const char *e = getenv("MTGP_JUMP_MODE"))
+        jump_mode = atoi(e) ? 1 : 0;
+    for (const auto &g : gens)
+        if (g.bits == 64 && c.class_of(g.slot) == 3)
+            jump_mode = 1;
+    // Two-wave cells (layout 2) per launch group: every generator of the
+    // group admits them and start states come from the batched jump launch
+    // (the two-wave kernel has no fused jump).
+    bool group_wide[7];
+    for (bool &w : group_wide)
+        w = jump_mode == 1;
+    for (const auto &g : gens)
+        if (!c.wide_ok(g.slot))
+            group_wide[(g.bits == 64 ? 3 : 0) + c.class_of(g.slot)] = false;
     // Resident workgroups of the default configuration of the largest class
     // among the plan's generators, and the numbers-weighted mean jump cost.
     int slots = 0;
@@ -804,17 +839,20 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     int active = 0;
     GridModel model;
     model.rate = 0;
+    model.cell_waves = 0;
     double w0 = 0;
     for (int g : named) {
         const GenRef &gr = gens[(size_t)g];
         const int b = gr.bits == 64, cls = c.class_of(gr.slot);
-        slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][c.layout[b][cls]]));
+        const int lay = c.layout_for(gr.bits, cls, group_wide[3 * b + cls]);
+        slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][lay]));
         const double nw = (double)per[(size_t)g];
         jcost += nw * jump_cost_numbers(gr.mexp, gr.n);
         const GridModel gm = grid_model_of(gr.bits, cls, gr.mexp, gr.n);
         model.jump_ns += nw * gm.jump_ns;
         model.rate += nw * gm.rate;
         model.bytes += nw * (gr.bits / 8);
+        model.cell_waves += nw * (lay == 2 ? 2 : 1);
         w0 += nw * gm.w0;
         weight += nw;
         active += per[(size_t)g] > 0;
@@ -824,8 +862,10 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         model.jump_ns /= weight;
         model.rate /= weight;
         model.w0 = w0 / weight;
+        model.cell_waves /= weight;
     } else {
         model.rate = 4700;
+        model.cell_waves = 1;
     }
     uint64_t S1 = 0, S2 = 0;
     double frac = 0;
@@ -924,16 +964,9 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     // (batched jumps, mode 1): from the cache when it has them (table
     // entries inline, one-product entries on host threads), else
     // incrementally along each generator on host threads.
-    if (opt.jump_mode >= 0)
-        jump_mode = opt.jump_mode ? 1 : 0;
-    else if (const char *e = getenv("MTGP_JUMP_MODE"))
-        jump_mode = atoi(e) ? 1 : 0;
     // Launch group of a generator: 0..2 = MTGP32 classes, 3..6 = MTGP64
-    // classes 0..3.  Narrow-class items need batched start states.
+    // classes 0..3.
     auto group_of = [&](const GenRef &g) { return (g.bits == 64 ? 3 : 0) + c.class_of(g.slot); };
-    for (const auto &g : gens)
-        if (g.bits == 64 && c.class_of(g.slot) == 3)
-            jump_mode = 1;
     const bool masks = jump_mode == 1;
     // Phase of a cell: 1 for the small cells of a two-size grid (batched
     // jumps only), else 0.  Phase 1 has launches of its own, which the
@@ -1232,12 +1265,13 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             L.cls = gi / 2 >= 3 ? gi / 2 - 3 : gi / 2;
             L.first = (int)i;
             L.count = (int)(j - i);
+            L.wide_ok = group_wide[gi / 2];
             if (jump) {
                 L.layout = 1;
                 L.grid = std::min(L.count, c.cus * c.jump_blocks_per_cu[L.bits == 64][L.cls]);
                 L.dyn_lds = 0;
             } else {
-                L.layout = c.layout[L.bits == 64][L.cls];
+                L.layout = c.layout_for(L.bits, L.cls, L.wide_ok);
                 const int cap = c.blocks_per_cu[L.bits == 64][L.cls][L.layout];
                 int bpc = std::min(c.bpc[L.bits == 64][L.cls], cap);
                 // Up to half a round more cells than slots (e.g. one cell per
@@ -1247,7 +1281,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
                 if (L.count > c.cus * bpc && L.count <= c.cus * cap && 2 * L.count <= 3 * c.cus * bpc)
                     bpc = (L.count + c.cus - 1) / c.cus;
                 L.grid = std::min(L.count, c.cus * bpc);
-                L.dyn_lds = c.lds_reserve(kernel_fn(L.bits, L.cls, L.layout), bpc);
+                L.dyn_lds = c.lds_reserve(kernel_fn(L.bits, L.cls, L.layout), kernel_threads(L.bits, L.cls, L.layout), bpc);
             }
             out.push_back(L);
             i = j;
@@ -1347,12 +1381,13 @@ void Plan::configure(int layout, int bpc) {
         if (layout >= 0) {
             if (layout >= kLayouts)
                 throw Error("unknown layout");
-            L.layout = layout;
+            // two-wave cells only where the launch admits them
+            L.layout = layout == 2 && !(L.wide_ok && layout_supported(L.bits, L.cls, 2)) ? 1 : layout;
         }
         int cap = ctx->blocks_per_cu[L.bits == 64][L.cls][L.layout];
         cap = std::min(cap, bpc > 0 ? bpc : ctx->bpc[L.bits == 64][L.cls]);
         L.grid = std::min(L.count, ctx->cus * cap);
-        L.dyn_lds = ctx->lds_reserve(kernel_fn(L.bits, L.cls, L.layout), cap);
+        L.dyn_lds = ctx->lds_reserve(kernel_fn(L.bits, L.cls, L.layout), kernel_threads(L.bits, L.cls, L.layout), cap);
     }
 }
 
@@ -1403,6 +1438,11 @@ float Plan::autotune(hipStream_t s, int reps, int *best_layout, int *best_bpc) {
     try {
         launch_part(0, s);  // start states stay valid for every trial
         for (int layout = 0; layout < kLayouts; layout++) {
+            bool any = layout != 2;
+            for (const auto &L : launches)
+                any = any || (L.wide_ok && layout_supported(L.bits, L.cls, 2));
+            if (!any)
+                continue;
             int last_cap = -1;
             for (int bpc : {4, 6, 8, 12, 16, 20, 24, 32}) {
                 int cap = bpc;
@@ -1437,7 +1477,8 @@ float Plan::autotune(hipStream_t s, int reps, int *best_layout, int *best_bpc) {
     (void)hipEventDestroy(e1);
     configure(bl, bb);
     for (const auto &L : launches) {
-        ctx->layout[L.bits == 64][L.cls] = bl;
+        if (L.layout == bl)
+            ctx->layout[L.bits == 64][L.cls] = bl;
         ctx->bpc[L.bits == 64][L.cls] = bb;
     }
     if (best_layout)

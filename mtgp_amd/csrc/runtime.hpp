@@ -35,9 +35,14 @@ struct Context {
     int n_params = 0, param_capacity = 0;
     // Occupancy limit [bits==64][class][layout] and the default layout
     // (class 3 exists for 64-bit only; kernels.hpp).
-    int blocks_per_cu[2][4][2] = {};
+    int blocks_per_cu[2][4][3] = {};  // [bits == 64][class][layout]
     int jump_blocks_per_cu[2][4] = {};
-    int layout[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
+    // Classes 1 and 2 default to two-wave cells (layout 2): on 64-set
+    // batches of MTGP32-23209 / MTGP32-44497 / MTGP64-44497 they measured
+    // 3.32 / 2.20 / 3.42 TB/s per step against 2.60 / 1.78 / 2.83 with
+    // single-wave cells (more waves per CU from fewer cells, so fewer
+    // start-state jumps).
+    int layout[2][4] = {{1, 2, 2, 1}, {1, 2, 2, 1}};
     // Resident workgroups per CU used by default.  The generation kernel is
     // held back by the board's power budget (its clock drops to 1.6-2.0 GHz
     // under full store traffic), so more waves do not generate faster but
@@ -104,11 +109,17 @@ struct Context {
     std::map<std::pair<const void *, int>, size_t> reserve_cache;
     // Dynamic LDS that limits `fn` to `per_cu` resident workgroups per CU
     // (0 when the kernel's own limit is already at most per_cu).
-    size_t lds_reserve(const void *fn, int per_cu);
+    size_t lds_reserve(const void *fn, int threads, int per_cu);
     int slot_of(const DevParam &rec, const unsigned char *sha1);
     const DevParam &param(int slot) const;
     // Kernel class of a parameter slot (from its N and pos).
     int class_of(int slot) const;
+    // Whether the slot's set admits two-wave cells (class 1 or 2, N - pos >=
+    // 512).
+    bool wide_ok(int slot) const;
+    // The default layout of (bits, class), with two-wave cells replaced by
+    // layout 1 where they are not allowed.
+    int layout_for(int bits, int cls, bool wide) const;
     void sha1_of(int slot, unsigned char out[20]) const;
 };
 
@@ -154,6 +165,7 @@ struct GridModel {
     double jump_ns = 0;   // chip time of one batched jump
     double rate = 4700;   // saturated generation rate, GB/s
     double w0 = 5.0;      // waves per CU at which the rate is 63% of it
+    double cell_waves = 1;  // waves per cell (2 for two-wave cells)
 };
 // Measured on MI355X per (word size, state class); classes without a
 // measurement use jump_ns = 4.5e-5 * MEXP * N * (bits / 32) / 2 (the jump's
@@ -235,6 +247,7 @@ struct Plan {
         int bits, cls, layout, first, count, grid;
         size_t dyn_lds;
         int phase;
+        bool wide_ok;  // layout 2 (two-wave cells) allowed for these items
     };
     Context *ctx = nullptr;
     // Start states of cells: 0 = jump inside the generation kernel

@@ -117,10 +117,11 @@ def test_jump_equals_stepping(mtgp, oracle, bits, mexp):
         assert_same(a, b, "s=%d" % s)
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 def test_genset_segmented_requests(mtgp, oracle, layout):
     """Several generators, requests with skips, split into short segments so
-    that most segments start from a device jump; both step layouts."""
+    that most segments start from a device jump; every step layout (layout
+    2, two-wave cells, applies to the MTGP32-23209 launch)."""
     specs = [(32, 11213, 3), (32, 23209, 9), (64, 23209, 4), (32, 11213, 100), (64, 44497, 2)]
     gs = mtgp.GenSet(specs)
     seeds = [11, 22, 33, 44, 55]
@@ -156,8 +157,8 @@ def test_genset_segmented_requests(mtgp, oracle, layout):
         assert_same(gs.state(g), windows[g][1], "state %d" % g)
 
 
-@pytest.mark.parametrize("layout", [0, 1])
-@pytest.mark.parametrize("bits,mexp", [(32, 11213), (64, 23209)])
+@pytest.mark.parametrize("layout", [0, 1, 2])
+@pytest.mark.parametrize("bits,mexp", [(32, 11213), (32, 44497), (64, 23209), (64, 44497)])
 def test_no_write_past_requested_outputs(mtgp, oracle, layout, bits, mexp):
     """Every request of a plan writes exactly its own range: guard words
     around each output stay untouched (tails of every length mod 256 and

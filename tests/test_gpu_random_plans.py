@@ -1,7 +1,7 @@
 """Randomised plans against the oracle: mixed word sizes and state classes,
 table and dynamic-creator sets (records), every output kind, short and
 long skips (stepped over or jumped), zero-count
-requests, automatic and explicit cell grids, both step layouts, the
+requests, automatic and explicit cell grids, every step layout, the
 two-size grid forced on small batches, the two-phase launch order, and a
 re-launch before the commit.  Every output number and every final window is
 compared bit for bit."""
@@ -94,7 +94,7 @@ def test_random_plan(mtgp, oracle, seed):
         plan = gs.plan(reqs, segment=segment)
     overlap = plan.set_overlap(rng.random() < 0.5)
     if rng.random() < 0.3:
-        plan.configure(layout=int(rng.integers(0, 2)))
+        plan.configure(layout=int(rng.integers(0, 3)))
     what = "seed %d: %d gens %s (records %s), %d requests, segment %d, two-size %s, two-phase %s, items %d" % (
         seed, G, specs, recs is not None, len(reqs), segment, force_two_size, overlap, plan.stats()["items"])
     buf.from_host(np.full(off + 64, 0xA5, np.uint8))

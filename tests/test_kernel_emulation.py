@@ -288,3 +288,59 @@ def test_narrow_class_items_and_jumps(emu, oracle, idx):
         want = oracle.jump(p, w0, oracle.poly_to_hex(q))
         got = jout[0, b] ^ jout[1, b]
         assert (got[1:] == want[1:]).all() and (got[0] & p.mask) == (want[0] & p.mask)
+
+
+@pytest.mark.parametrize("bits,mexp", [(32, 23209), (32, 44497), (64, 44497), (64, 110503)])
+def test_two_wave_cells(emu, oracle, bits, mexp):
+    """Layout 2 (two waves per cell, 512-word steps): every kind with counts
+    around the step size, unaligned outputs, a start window from partial
+    windows, a segment chain with numbers stepped over, and the queue with
+    fewer workgroups than items."""
+    p = oracle.params(bits, mexp, 3)
+    assert p.n - p.pos >= 512
+    eb = bits // 8
+    wd = np.uint32 if bits == 32 else np.uint64
+    w0 = oracle.init_state(p, 99)
+    counts = [(0, 1), (1, 511), (2, 513), (3, 1030), (4, 3000), (0, 0), (2, 4099)]
+    items, keep, want = [], [], []
+    for k, (kind, n) in enumerate(counts):
+        raw = np.zeros((n + 16) * eb + 128, np.uint8)
+        base = raw.ctypes.data + (-raw.ctypes.data) % 128 + eb * (7 * k % 16)
+        sin, sout = w0.copy(), np.zeros_like(w0)
+        items.append(dict(out=base, count=n, state_in=sin.ctypes.data, state_out=sout.ctypes.data,
+                          param=pyemu.set_id(bits, mexp, 3), kind=kind, n=p.n))
+        keep.append((raw, base, sin, sout, n))
+        want.append(oracle.generate(p, w0, kind, n))
+    # start window as the XOR of three partial windows
+    rng = np.random.default_rng(mexp)
+    parts = rng.integers(0, np.iinfo(wd).max, (3, p.n), dtype=wd)
+    parts[2] = w0 ^ parts[0] ^ parts[1]
+    plan = [(0, 5), (None, 700), (2, 1500), (None, 3), (1, 513)]
+    segs = np.zeros(len(plan), pyemu.SEGMENT)
+    chain, w = [], w0
+    for k, (kind, n) in enumerate(plan):
+        if kind is None:
+            segs[k] = (0, n, 0, 0, 0)
+            _, w = oracle.generate(p, w, 0, n)
+            continue
+        raw = np.zeros((n + 40) * eb + 256, np.uint8)
+        base = raw.ctypes.data + (-raw.ctypes.data) % 128 + eb * (5 * k % 16)
+        segs[k] = (base, n, kind, 0, 0)
+        e, w = oracle.generate(p, w, kind, n)
+        chain.append((raw, base, n, e))
+    cout = np.zeros_like(w0)
+    items.append(dict(state_in=parts.ctypes.data, state_out=cout.ctypes.data, param=pyemu.set_id(bits, mexp, 3),
+                      segs=segs.ctypes.data, nsegs=len(plan), parts=3, part_bytes=p.n * eb, n=p.n))
+    _run(emu, bits, items, 2, grid=3)
+    for (raw, base, sin, sout, n), (e, we) in zip(keep, want):
+        start = base - raw.ctypes.data
+        got = raw[start:start + n * eb].view(e.dtype)
+        assert (_bits(got) == _bits(e)).all()
+        assert (raw[start + n * eb:] == 0).all() and (raw[:start] == 0).all(), "write outside the outputs"
+        assert (sout == we).all()
+    for raw, base, n, e in chain:
+        start = base - raw.ctypes.data
+        got = raw[start:start + n * eb].view(e.dtype)
+        assert (_bits(got) == _bits(e)).all()
+        assert (raw[start + n * eb:] == 0).all() and (raw[:start] == 0).all()
+    assert (cout == w).all()
