@@ -5,7 +5,8 @@
 // jump of openCL-sample/mtgp32-jump.cl:630-731) with one persistent kernel
 // per word size that works through a list of Items (common.hpp).
 //
-// Design (one workgroup = one wave64 = one Item at a time):
+// Design (one workgroup = one wave64 = one Item at a time; the two-wave
+// cells of layout 2 are the exception, see mtgp32_items_kernel):
 //  * The state window of N words lives in a linear LDS buffer.  A step
 //    appends 256 words: lane l computes words 4l..4l+3, which are mutually
 //    independent because every parameter set has N - pos >= 256.  The step
@@ -50,7 +51,8 @@ constexpr int kStep = 256;  // words per wave step (4 per lane)
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Single-wave workgroup: orders this wave's LDS accesses across steps.
+// Orders the workgroup's LDS accesses across steps (a barrier of one wave,
+// or of the two waves of a layout-2 cell).
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // The 16-entry tables (recursion tbl, tempering tmp, float tempering flt)
