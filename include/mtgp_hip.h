@@ -187,10 +187,11 @@ int mtgp_plan_commit(mtgp_plan_t *plan);
 void mtgp_plan_destroy(mtgp_plan_t *plan);
 /* Tuning knobs (the autotuner's): step layout (0 = four consecutive
  * numbers per lane, 128-bit stores; 1 = lane-strided, conflict-free LDS;
- * 2 = layout 1 with two waves per cell and 512-word steps, for states of
- * MEXP 23209 or more (MTGP32) and 44497 or more (MTGP64) whose sets all
- * have N - pos >= 512 and whose start states come from the batched jump
- * launch -- other launches of the plan take layout 1) and resident workgroups per CU; -1 leaves a knob
+ * 2 = layout 1 with two waves per cell: 512-word steps, or 256-word
+ * steps of two words per lane for MTGP32-11213 and MTGP64-23209; for
+ * launches whose sets all have N - pos of at least that step and whose
+ * start states come from the batched jump launch -- other launches of the
+ * plan take layout 1) and resident workgroups per CU; -1 leaves a knob
  * unchanged. */
 int mtgp_plan_configure(mtgp_plan_t *plan, int layout, int blocks_per_cu);
 /* Two-phase launch order (off by default; MTGP_OVERLAP=1 changes the

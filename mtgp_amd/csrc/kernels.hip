@@ -155,34 +155,35 @@ __device__ __forceinline__ void step32(uint32_t *buf, const uint32_t *tab, P32 p
 // Lane-strided variant of step32: lane l computes words l + 64i (i < 4),
 // so every LDS access of the step is lane-consecutive (no bank conflicts)
 // and every store instruction writes 256 contiguous bytes.  No alignment
-// requirement on b or dst.  With W waves per workgroup the step is 256W
-// words (thread l computes l + 64Wi), which needs N - pos >= 256W.
-template <int KIND, bool OUT, bool FULL = false, int W = 1>
+// requirement on b or dst.  With W waves per workgroup and WPL words per
+// lane the step is 64*W*WPL words (thread l computes l + 64Wi, i < WPL),
+// which needs N - pos >= 64*W*WPL.
+template <int KIND, bool OUT, bool FULL = false, int W = 1, int WPL = 4>
 __device__ __forceinline__ void step32s(uint32_t *buf, const uint32_t *tab, P32 p, int b,
                                         gu32 *dst, uint32_t m) {
     constexpr int T = kWave * W;
     const int l = threadIdx.x;
     const uint32_t *xp = buf + b + l;
     const uint32_t *yp = xp + p.pm1;
-    uint32_t r[4], t[4];
+    uint32_t r[WPL], t[WPL];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < WPL; i++) {
         t[i] = yp[T * i];
         r[i] = rec32(p, tab, xp[T * i], xp[T * i + 1], yp[T * i + 1]);
     }
     uint32_t *wp = buf + b + p.n + l;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < WPL; i++)
         wp[T * i] = r[i];
     if (OUT) {
-        uint32_t o[4];
+        uint32_t o[WPL];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < WPL; i++)
             o[i] = out32<KIND>(tab, r[i], t[i]);
         // the m leading outputs: lanes past them fall outside the window
-        const store_window_t w = store_window((uint64_t)(uintptr_t)dst, FULL ? 4u * kStep * W : 4u * m);
+        const store_window_t w = store_window((uint64_t)(uintptr_t)dst, FULL ? 4u * T * WPL : 4u * m);
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < WPL; i++)
             store_u32(w, 4u * (uint32_t)(l + T * i), o[i]);
     }
 }
@@ -290,11 +291,11 @@ __device__ __forceinline__ uint32_t head32(uint64_t out, uint64_t count) {
 
 // Window base for a segment with head h: layout 0 needs (b + h) % 4 == 0;
 // both need room for one more step.  Moves the window when either fails.
-template <int NJ, int NBUF, int LAYOUT, int W = 1>
+template <int NJ, int NBUF, int LAYOUT, int W = 1, int WPL = 4>
 __device__ __forceinline__ int place32(uint32_t *buf, P32 p, int b, uint32_t h) {
     const int to = LAYOUT == 0 ? (int)((4u - h) & 3u) : 0;
     const bool misaligned = LAYOUT == 0 && (((uint32_t)b + h) & 3u) != 0;
-    if (misaligned || b + p.n + kStep * W > NBUF) {
+    if (misaligned || b + p.n + kWave * W * WPL > NBUF) {
         relocate32<NJ, W>(buf, b, p.n, to);
         return to;
     }
@@ -302,15 +303,15 @@ __device__ __forceinline__ int place32(uint32_t *buf, P32 p, int b, uint32_t h) 
 }
 
 // Steps the window over `count` numbers without output; returns the new base.
-template <int NJ, int NBUF, int W = 1>
+template <int NJ, int NBUF, int W = 1, int WPL = 4>
 __device__ __forceinline__ int advance32(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t count) {
-    constexpr int S = kStep * W;
+    constexpr int S = kWave * W * WPL;
     for (uint64_t s = count / S; s > 0; s--) {
         if (b + p.n + S > NBUF) {
             relocate32<NJ, W>(buf, b, p.n);
             b = 0;
         }
-        step32s<0, false, false, W>(buf, tab, p, b, nullptr, 0);
+        step32s<0, false, false, W, WPL>(buf, tab, p, b, nullptr, 0);
         wave_sync();
         b += S;
     }
@@ -320,7 +321,7 @@ __device__ __forceinline__ int advance32(uint32_t *buf, const uint32_t *tab, P32
             relocate32<NJ, W>(buf, b, p.n);
             b = 0;
         }
-        step32s<0, false, false, W>(buf, tab, p, b, nullptr, 0);
+        step32s<0, false, false, W, WPL>(buf, tab, p, b, nullptr, 0);
         wave_sync();
         b += (int)m;
     }
@@ -329,11 +330,11 @@ __device__ __forceinline__ int advance32(uint32_t *buf, const uint32_t *tab, P32
 
 // Writes `count` numbers of KIND to out from the window at base b (placed
 // by place32 for head h); returns the window base after the last number.
-template <int KIND, int NJ, int NBUF, int LAYOUT, int W = 1>
+template <int KIND, int NJ, int NBUF, int LAYOUT, int W = 1, int WPL = 4>
 __device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint32_t h,
                           gu32 *__restrict__ out, uint64_t count) {
-    static_assert(W == 1 || LAYOUT == 1, "multi-wave cells are lane-strided");
-    constexpr int S = kStep * W;
+    static_assert(LAYOUT == 1 || (W == 1 && WPL == 4), "multi-wave cells are lane-strided");
+    constexpr int S = kWave * W * WPL;
     const int l = threadIdx.x;
     gu32 *dst = out;
     uint64_t rest = count;
@@ -350,7 +351,7 @@ __device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P3
         } else {
             // h < 32 numbers so that every later 256-byte store starts on a
             // 128-byte line (no partially written lines).
-            step32s<KIND, true, false, W>(buf, tab, p, b, dst, h);
+            step32s<KIND, true, false, W, WPL>(buf, tab, p, b, dst, h);
         }
         wave_sync();
         b += (int)h;
@@ -367,7 +368,7 @@ __device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P3
         if (LAYOUT == 0)
             step32<KIND, true, true>(buf, tab, p, b, dst, kStep);
         else
-            step32s<KIND, true, true, W>(buf, tab, p, b, dst, S);
+            step32s<KIND, true, true, W, WPL>(buf, tab, p, b, dst, S);
         wave_sync();
         b += S;
         dst += S;
@@ -381,7 +382,7 @@ __device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P3
         if (LAYOUT == 0)
             step32<KIND, true>(buf, tab, p, b, dst, m);
         else
-            step32s<KIND, true, false, W>(buf, tab, p, b, dst, m);
+            step32s<KIND, true, false, W, WPL>(buf, tab, p, b, dst, m);
         wave_sync();
         b += (int)m;
     }
@@ -389,52 +390,53 @@ __device__ __forceinline__ int generate32(uint32_t *buf, const uint32_t *tab, P3
 }
 
 // One segment (Segment, common.hpp) from window base b; returns the new base.
-template <int NJ, int NBUF, int LAYOUT, int W = 1>
+template <int NJ, int NBUF, int LAYOUT, int W = 1, int WPL = 4>
 __device__ __forceinline__ int segment32_body(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t out,
                                               uint64_t count, int kind) {
     if (count == 0)
         return b;
     if (out == 0)
-        return advance32<NJ, NBUF, W>(buf, tab, p, b, count);
+        return advance32<NJ, NBUF, W, WPL>(buf, tab, p, b, count);
     const uint32_t h = head32<LAYOUT>(out, count);
-    b = place32<NJ, NBUF, LAYOUT, W>(buf, p, b, h);
+    b = place32<NJ, NBUF, LAYOUT, W, WPL>(buf, p, b, h);
     gu32 *dst = reinterpret_cast<gu32 *>(out);
     switch (kind) {
-    case 0: return generate32<0, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
-    case 1: return generate32<1, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
-    case 2: return generate32<2, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
-    case 3: return generate32<3, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
-    default: return generate32<4, NJ, NBUF, LAYOUT, W>(buf, tab, p, b, h, dst, count);
+    case 0: return generate32<0, NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, h, dst, count);
+    case 1: return generate32<1, NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, h, dst, count);
+    case 2: return generate32<2, NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, h, dst, count);
+    case 3: return generate32<3, NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, h, dst, count);
+    default: return generate32<4, NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, h, dst, count);
     }
 }
 
 // The small state classes (NJ <= 6) inline the segment into the kernel, so
 // that LDS offsets fold into the instructions; the large ones call an
 // out-of-line copy (fully inlined, their kernels spill registers).
-template <int NJ, int NBUF, int LAYOUT, int W>
+template <int NJ, int NBUF, int LAYOUT, int W, int WPL>
 __device__ __noinline__ int segment32_call(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t out,
                                            uint64_t count, int kind) {
-    return segment32_body<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, out, count, kind);
+    return segment32_body<NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, out, count, kind);
 }
-template <int NJ, int NBUF, int LAYOUT, int W = 1>
+template <int NJ, int NBUF, int LAYOUT, int W = 1, int WPL = 4>
 __device__ __forceinline__ int segment32(uint32_t *buf, const uint32_t *tab, P32 p, int b, uint64_t out,
                                          uint64_t count, int kind) {
     if constexpr (NJ <= 6)
-        return segment32_body<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, out, count, kind);
+        return segment32_body<NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, out, count, kind);
     else
-        return segment32_call<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, out, count, kind);
+        return segment32_call<NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, out, count, kind);
 }
 
-// W waves per workgroup (W > 1: lane-strided only, every set of the
-// class has N - pos >= 256W, and no Item carries a fused jump polynomial;
-// the planner guarantees all three).
-template <int NMAX, int K, int LAYOUT, int W = 1>
+// W waves per workgroup, WPL words per lane and step (W > 1: lane-strided
+// only, every set of the class has N - pos >= 64*W*WPL, and no Item carries
+// a fused jump polynomial; the planner guarantees all three).
+template <int NMAX, int K, int LAYOUT, int W = 1, int WPL = 4>
 __global__ __launch_bounds__(64 * W) void mtgp32_items_kernel(const Item *__restrict__ items, int n_items,
                                                               const DevParam *__restrict__ params,
                                                               uint32_t *__restrict__ queue) {
+    static_assert(W == 1 || WPL == 4 || WPL == 2, "words per lane");
     constexpr int T = kWave * W;
     constexpr int NJ = (NMAX + T - 1) / T;
-    constexpr int NBUF = K * kStep * W + T * NJ + T;
+    constexpr int NBUF = K * T * WPL + T * NJ + T;
     __shared__ __attribute__((aligned(16))) uint32_t buf[NBUF];
     __shared__ uint32_t tab[kTabWords];
     __shared__ uint32_t next_item;
@@ -482,9 +484,10 @@ __global__ __launch_bounds__(64 * W) void mtgp32_items_kernel(const Item *__rest
         if (I.nsegs > 0) {
             const GMEM Segment *segs = reinterpret_cast<const GMEM Segment *>(I.segs);
             for (int si = 0; si < I.nsegs; si++)
-                b = segment32<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, segs[si].out, segs[si].count, segs[si].kind);
+                b = segment32<NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, segs[si].out, segs[si].count,
+                                                        segs[si].kind);
         } else {
-            b = segment32<NJ, NBUF, LAYOUT, W>(buf, tab, p, b, I.out, I.count, I.kind);
+            b = segment32<NJ, NBUF, LAYOUT, W, WPL>(buf, tab, p, b, I.out, I.count, I.kind);
         }
         if (state_out)
             for (int j = l; j < p.n; j += T)
@@ -1204,16 +1207,20 @@ constexpr int kNarrowN = 176;
 
 template <int B, int NMAX, int LAYOUT, int WPL = 4>
 struct KernelOf;
-// MTGP32 layout 2: lane-strided two-wave cells (512-word steps).
+// Layout 2: lane-strided two-wave cells, 512-word steps (256-word steps of
+// two words per lane for the N <= 351/363 classes, whose N - pos can be as
+// small as 258).
 template <int NMAX, int LAYOUT>
 struct KernelOf<32, NMAX, LAYOUT, 4> {
     static constexpr int waves = LAYOUT == 2 ? 2 : 1;
-    static constexpr auto fn = mtgp32_items_kernel<NMAX, kK, LAYOUT == 2 ? 1 : LAYOUT, waves>;
+    static constexpr auto fn =
+        mtgp32_items_kernel<NMAX, kK, LAYOUT == 2 ? 1 : LAYOUT, waves, LAYOUT == 2 && NMAX <= 351 ? 2 : 4>;
 };
 template <int NMAX, int LAYOUT, int WPL>
 struct KernelOf<64, NMAX, LAYOUT, WPL> {
     static constexpr int waves = LAYOUT == 2 ? 2 : 1;
-    static constexpr auto fn = mtgp64_items_kernel<NMAX, kK, LAYOUT == 2 ? 1 : LAYOUT, WPL, waves>;
+    static constexpr auto fn =
+        mtgp64_items_kernel<NMAX, kK, LAYOUT == 2 ? 1 : LAYOUT, LAYOUT == 2 && NMAX <= 363 ? 2 : WPL, waves>;
 };
 
 template <class KO>
@@ -1235,8 +1242,8 @@ auto with_kernel(int bits, int cls, int layout, F &&f) {
     if (bits == B && N_OF(B, cls) == N && layout == L) return f.template operator()<KernelOf<B, N, L>>();
 #define N_OF(B, c) ((B) == 32 ? ((c) == 0 ? 351 : (c) == 1 ? 726 : 1391) : ((c) == 0 ? 363 : (c) == 1 ? 696 : 1727))
     MTGP_CASE(32, 351, 0) MTGP_CASE(32, 351, 1) MTGP_CASE(32, 726, 0) MTGP_CASE(32, 726, 1)
-    MTGP_CASE(32, 1391, 0) MTGP_CASE(32, 1391, 1) MTGP_CASE(32, 726, 2) MTGP_CASE(32, 1391, 2)
-    MTGP_CASE(64, 363, 0) MTGP_CASE(64, 363, 1)
+    MTGP_CASE(32, 1391, 0) MTGP_CASE(32, 1391, 1) MTGP_CASE(32, 351, 2) MTGP_CASE(32, 726, 2)
+    MTGP_CASE(32, 1391, 2) MTGP_CASE(64, 363, 0) MTGP_CASE(64, 363, 1) MTGP_CASE(64, 363, 2)
     MTGP_CASE(64, 696, 0) MTGP_CASE(64, 696, 1) MTGP_CASE(64, 1727, 0) MTGP_CASE(64, 1727, 1)
     MTGP_CASE(64, 696, 2) MTGP_CASE(64, 1727, 2)
 #undef MTGP_CASE
@@ -1275,7 +1282,7 @@ bool layout_supported(int bits, int cls, int layout) {
     if (bits == 64 && cls == 3)
         return layout == 1;
     if (layout == 2)
-        return cls == 1 || cls == 2;
+        return cls <= 2;
     return true;
 }
 

@@ -16,8 +16,9 @@ constexpr int kClasses = 4;  // classes 0..2 of both word sizes, 3 for 64-bit
 // Step layouts: 0 = each lane computes 4 consecutive words and stores them
 // with one 128-bit store; 1 = lane l computes words l + 64i (conflict-free
 // LDS, 256-byte coalesced dword stores); 2 = layout 1 with two waves per
-// cell, thread l computing words l + 128i of 512-word steps (classes 1 and
-// 2 of both word sizes; needs N - pos >= 512 and batched start states).
+// cell, thread l computing words l + 128i of 512-word steps (256-word steps
+// of two words per lane for class 0); needs N - pos of at least the step
+// and batched start states.
 constexpr int kLayouts = 3;
 // Whether (bits, cls) has a kernel of that layout.
 bool layout_supported(int bits, int cls, int layout);

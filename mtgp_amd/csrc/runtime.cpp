@@ -230,8 +230,10 @@ int Context::class_of(int slot) const {
 }
 
 bool Context::wide_ok(int slot) const {
+    // A two-wave cell steps 512 words, or 256 for class 0 (two per lane).
     const DevParam &d = param(slot);
-    return d.n - d.pos >= 512 && layout_supported(d.bits, kernel_class(d.bits, d.n, d.pos), 2);
+    const int cls = kernel_class(d.bits, d.n, d.pos);
+    return layout_supported(d.bits, cls, 2) && d.n - d.pos >= (cls == 0 ? 256 : 512);
 }
 
 int Context::layout_for(int bits, int cls, bool wide) const {

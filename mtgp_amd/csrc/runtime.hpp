@@ -37,12 +37,14 @@ struct Context {
     // (class 3 exists for 64-bit only; kernels.hpp).
     int blocks_per_cu[2][4][3] = {};  // [bits == 64][class][layout]
     int jump_blocks_per_cu[2][4] = {};
-    // Classes 1 and 2 default to two-wave cells (layout 2): on 64-set
-    // batches of MTGP32-23209 / MTGP32-44497 / MTGP64-44497 they measured
-    // 3.32 / 2.20 / 3.42 TB/s per step against 2.60 / 1.78 / 2.83 with
-    // single-wave cells (more waves per CU from fewer cells, so fewer
-    // start-state jumps).
-    int layout[2][4] = {{1, 2, 2, 1}, {1, 2, 2, 1}};
+    // Two-wave cells (layout 2) by default where they measured faster: on
+    // 64-set batches of MTGP32-23209 / MTGP32-44497 / MTGP64-23209 /
+    // MTGP64-44497, 3.32 / 2.20 / 4.09 / 3.42 TB/s per step against 2.60 /
+    // 1.78 / 3.86-3.95 / 2.83 with single-wave cells (more waves per CU from
+    // fewer cells, so fewer start-state jumps).  Batched MTGP32-11213 keeps
+    // single-wave cells: two-wave ones measured 4.16 / 4.25 / 4.04 TB/s at 6
+    // / 8 / 12 per CU against 4.21.
+    int layout[2][4] = {{1, 2, 2, 1}, {2, 2, 2, 1}};
     // Resident workgroups per CU used by default.  The generation kernel is
     // held back by the board's power budget (its clock drops to 1.6-2.0 GHz
     // under full store traffic), so more waves do not generate faster but
@@ -114,8 +116,8 @@ struct Context {
     const DevParam &param(int slot) const;
     // Kernel class of a parameter slot (from its N and pos).
     int class_of(int slot) const;
-    // Whether the slot's set admits two-wave cells (class 1 or 2, N - pos >=
-    // 512).
+    // Whether the slot's set admits two-wave cells (classes 0-2 with N - pos
+    // at least their 512-word step, 256 for class 0).
     bool wide_ok(int slot) const;
     // The default layout of (bits, class), with two-wave cells replaced by
     // layout 1 where they are not allowed.
