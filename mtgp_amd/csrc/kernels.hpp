@@ -17,11 +17,13 @@ constexpr int kClasses = 4;  // classes 0..2 of both word sizes, 3 for 64-bit
 // with one 128-bit store; 1 = lane l computes words l + 64i (conflict-free
 // LDS, 256-byte coalesced dword stores); 2 = layout 1 with two waves per
 // cell, thread l computing words l + 128i of 512-word steps (256-word steps
-// of two words per lane for class 0); needs N - pos of at least the step
-// and batched start states.
-constexpr int kLayouts = 3;
-// Whether (bits, cls) has a kernel of that layout.
+// of two words per lane for class 0); 3 = four waves per cell, 1024-word
+// steps (class 2).  Layouts 2 and 3 need N - pos of at least the step and
+// batched start states.
+constexpr int kLayouts = 4;
+// Whether (bits, cls) has a kernel of that layout, and its words per step.
 bool layout_supported(int bits, int cls, int layout);
+int layout_step_words(int bits, int cls, int layout);
 // Kernel entry (for occupancy queries) and its workgroup size.
 const void *kernel_fn(int bits, int cls, int layout);
 int kernel_threads(int bits, int cls, int layout);

@@ -229,16 +229,24 @@ int Context::class_of(int slot) const {
     return kernel_class(d.bits, d.n, d.pos);
 }
 
-bool Context::wide_ok(int slot) const {
-    // A two-wave cell steps 512 words, or 256 for class 0 (two per lane).
+int Context::layouts_ok(int slot) const {
     const DevParam &d = param(slot);
     const int cls = kernel_class(d.bits, d.n, d.pos);
-    return layout_supported(d.bits, cls, 2) && d.n - d.pos >= (cls == 0 ? 256 : 512);
+    int mask = 0;
+    for (int lay = 0; lay < kLayouts; lay++)
+        if (lay < 2 || (layout_supported(d.bits, cls, lay) && d.n - d.pos >= layout_step_words(d.bits, cls, lay)))
+            mask |= 1 << lay;
+    return mask;
 }
 
-int Context::layout_for(int bits, int cls, bool wide) const {
-    const int lay = layout[bits == 64][cls];
-    return lay == 2 && !(wide && layout_supported(bits, cls, 2)) ? 1 : lay;
+int Context::layout_within(int lay, int mask) {
+    while (lay >= 2 && !(mask >> lay & 1))
+        lay--;
+    return lay;
+}
+
+int Context::layout_for(int bits, int cls, int mask) const {
+    return layout_within(layout[bits == 64][cls], mask);
 }
 
 const DevParam &Context::param(int slot) const {
@@ -825,15 +833,14 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     for (const auto &g : gens)
         if (g.bits == 64 && c.class_of(g.slot) == 3)
             jump_mode = 1;
-    // Two-wave cells (layout 2) per launch group: every generator of the
-    // group admits them and start states come from the batched jump launch
-    // (the two-wave kernel has no fused jump).
-    bool group_wide[7];
-    for (bool &w : group_wide)
-        w = jump_mode == 1;
+    // Layouts per launch group: multi-wave cells (layouts 2 and 3) where
+    // every generator of the group admits them and start states come from
+    // the batched jump launch (the multi-wave kernels have no fused jump).
+    int group_ok[7];
+    for (int &m : group_ok)
+        m = jump_mode == 1 ? (1 << kLayouts) - 1 : 3;
     for (const auto &g : gens)
-        if (!c.wide_ok(g.slot))
-            group_wide[(g.bits == 64 ? 3 : 0) + c.class_of(g.slot)] = false;
+        group_ok[(g.bits == 64 ? 3 : 0) + c.class_of(g.slot)] &= c.layouts_ok(g.slot);
     // Resident workgroups of the default configuration of the largest class
     // among the plan's generators, and the numbers-weighted mean jump cost.
     int slots = 0;
@@ -846,7 +853,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     for (int g : named) {
         const GenRef &gr = gens[(size_t)g];
         const int b = gr.bits == 64, cls = c.class_of(gr.slot);
-        const int lay = c.layout_for(gr.bits, cls, group_wide[3 * b + cls]);
+        const int lay = c.layout_for(gr.bits, cls, group_ok[3 * b + cls]);
         slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][lay]));
         const double nw = (double)per[(size_t)g];
         jcost += nw * jump_cost_numbers(gr.mexp, gr.n);
@@ -854,7 +861,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         model.jump_ns += nw * gm.jump_ns;
         model.rate += nw * gm.rate;
         model.bytes += nw * (gr.bits / 8);
-        model.cell_waves += nw * (lay == 2 ? 2 : 1);
+        model.cell_waves += nw * (lay == 3 ? 4 : lay == 2 ? 2 : 1);
         w0 += nw * gm.w0;
         weight += nw;
         active += per[(size_t)g] > 0;
@@ -1267,13 +1274,13 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             L.cls = gi / 2 >= 3 ? gi / 2 - 3 : gi / 2;
             L.first = (int)i;
             L.count = (int)(j - i);
-            L.wide_ok = group_wide[gi / 2];
+            L.layouts_ok = group_ok[gi / 2];
             if (jump) {
                 L.layout = 1;
                 L.grid = std::min(L.count, c.cus * c.jump_blocks_per_cu[L.bits == 64][L.cls]);
                 L.dyn_lds = 0;
             } else {
-                L.layout = c.layout_for(L.bits, L.cls, L.wide_ok);
+                L.layout = c.layout_for(L.bits, L.cls, L.layouts_ok);
                 const int cap = c.blocks_per_cu[L.bits == 64][L.cls][L.layout];
                 int bpc = std::min(c.bpc[L.bits == 64][L.cls], cap);
                 // Up to half a round more cells than slots (e.g. one cell per
@@ -1383,8 +1390,8 @@ void Plan::configure(int layout, int bpc) {
         if (layout >= 0) {
             if (layout >= kLayouts)
                 throw Error("unknown layout");
-            // two-wave cells only where the launch admits them
-            L.layout = layout == 2 && !(L.wide_ok && layout_supported(L.bits, L.cls, 2)) ? 1 : layout;
+            // multi-wave cells only where the launch admits them
+            L.layout = Context::layout_within(layout, L.layouts_ok);
         }
         int cap = ctx->blocks_per_cu[L.bits == 64][L.cls][L.layout];
         cap = std::min(cap, bpc > 0 ? bpc : ctx->bpc[L.bits == 64][L.cls]);
@@ -1440,9 +1447,9 @@ float Plan::autotune(hipStream_t s, int reps, int *best_layout, int *best_bpc) {
     try {
         launch_part(0, s);  // start states stay valid for every trial
         for (int layout = 0; layout < kLayouts; layout++) {
-            bool any = layout != 2;
+            bool any = layout < 2;
             for (const auto &L : launches)
-                any = any || (L.wide_ok && layout_supported(L.bits, L.cls, 2));
+                any = any || (L.layouts_ok >> layout & 1);
             if (!any)
                 continue;
             int last_cap = -1;

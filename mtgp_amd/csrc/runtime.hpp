@@ -35,16 +35,17 @@ struct Context {
     int n_params = 0, param_capacity = 0;
     // Occupancy limit [bits==64][class][layout] and the default layout
     // (class 3 exists for 64-bit only; kernels.hpp).
-    int blocks_per_cu[2][4][3] = {};  // [bits == 64][class][layout]
+    int blocks_per_cu[2][4][4] = {};  // [bits == 64][class][layout]
     int jump_blocks_per_cu[2][4] = {};
-    // Two-wave cells (layout 2) by default where they measured faster: on
-    // 64-set batches of MTGP32-23209 / MTGP32-44497 / MTGP64-23209 /
-    // MTGP64-44497, 3.32 / 2.20 / 4.09 / 3.42 TB/s per step against 2.60 /
-    // 1.78 / 3.86-3.95 / 2.83 with single-wave cells (more waves per CU from
-    // fewer cells, so fewer start-state jumps).  Batched MTGP32-11213 keeps
+    // Multi-wave cells by default where they measured faster: on 64-set
+    // batches of MTGP32-23209 / MTGP64-23209 / MTGP64-44497, two-wave cells
+    // (layout 2) 3.32 / 4.09 / 3.42 TB/s per step against 2.60 / 3.86-3.95 /
+    // 2.83 with single-wave cells; MTGP32-44497, four-wave cells (layout 3)
+    // 2.86 against 2.20 (two-wave) and 1.78 (more waves per CU from fewer
+    // cells, so fewer start-state jumps).  Batched MTGP32-11213 keeps
     // single-wave cells: two-wave ones measured 4.16 / 4.25 / 4.04 TB/s at 6
     // / 8 / 12 per CU against 4.21.
-    int layout[2][4] = {{1, 2, 2, 1}, {2, 2, 2, 1}};
+    int layout[2][4] = {{1, 2, 3, 1}, {2, 2, 2, 1}};
     // Resident workgroups per CU used by default.  The generation kernel is
     // held back by the board's power budget (its clock drops to 1.6-2.0 GHz
     // under full store traffic), so more waves do not generate faster but
@@ -116,12 +117,14 @@ struct Context {
     const DevParam &param(int slot) const;
     // Kernel class of a parameter slot (from its N and pos).
     int class_of(int slot) const;
-    // Whether the slot's set admits two-wave cells (classes 0-2 with N - pos
-    // at least their 512-word step, 256 for class 0).
-    bool wide_ok(int slot) const;
-    // The default layout of (bits, class), with two-wave cells replaced by
-    // layout 1 where they are not allowed.
-    int layout_for(int bits, int cls, bool wide) const;
+    // Layouts the slot's set admits, as a bit mask (bit l: layout l); the
+    // multi-wave layouts need N - pos of at least their step.
+    int layouts_ok(int slot) const;
+    // `lay` where the mask allows it, else the next narrower layout (3 -> 2
+    // -> 1).
+    static int layout_within(int lay, int mask);
+    // The default layout of (bits, class) within the mask.
+    int layout_for(int bits, int cls, int mask) const;
     void sha1_of(int slot, unsigned char out[20]) const;
 };
 
@@ -249,7 +252,7 @@ struct Plan {
         int bits, cls, layout, first, count, grid;
         size_t dyn_lds;
         int phase;
-        bool wide_ok;  // layout 2 (two-wave cells) allowed for these items
+        int layouts_ok;  // bit mask of the layouts allowed for these items
     };
     Context *ctx = nullptr;
     // Start states of cells: 0 = jump inside the generation kernel

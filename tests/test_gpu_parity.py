@@ -157,7 +157,7 @@ def test_genset_segmented_requests(mtgp, oracle, layout):
         assert_same(gs.state(g), windows[g][1], "state %d" % g)
 
 
-@pytest.mark.parametrize("layout", [0, 1, 2])
+@pytest.mark.parametrize("layout", [0, 1, 2, 3])
 @pytest.mark.parametrize("bits,mexp", [(32, 11213), (32, 44497), (64, 23209), (64, 44497)])
 def test_no_write_past_requested_outputs(mtgp, oracle, layout, bits, mexp):
     """Every request of a plan writes exactly its own range: guard words

@@ -94,7 +94,7 @@ def test_random_plan(mtgp, oracle, seed):
         plan = gs.plan(reqs, segment=segment)
     overlap = plan.set_overlap(rng.random() < 0.5)
     if rng.random() < 0.3:
-        plan.configure(layout=int(rng.integers(0, 3)))
+        plan.configure(layout=int(rng.integers(0, 4)))
     what = "seed %d: %d gens %s (records %s), %d requests, segment %d, two-size %s, two-phase %s, items %d" % (
         seed, G, specs, recs is not None, len(reqs), segment, force_two_size, overlap, plan.stats()["items"])
     buf.from_host(np.full(off + 64, 0xA5, np.uint8))

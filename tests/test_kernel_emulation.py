@@ -290,16 +290,17 @@ def test_narrow_class_items_and_jumps(emu, oracle, idx):
         assert (got[1:] == want[1:]).all() and (got[0] & p.mask) == (want[0] & p.mask)
 
 
-@pytest.mark.parametrize("bits,mexp", [(32, 11213), (32, 23209), (32, 44497), (64, 23209), (64, 44497),
-                                       (64, 110503)])
-def test_two_wave_cells(emu, oracle, bits, mexp):
+@pytest.mark.parametrize("bits,mexp,layout", [(32, 11213, 2), (32, 23209, 2), (32, 44497, 2), (64, 23209, 2),
+                                              (64, 44497, 2), (64, 110503, 2), (32, 44497, 3), (64, 110503, 3)])
+def test_multi_wave_cells(emu, oracle, bits, mexp, layout):
     """Layout 2 (two waves per cell; 512-word steps, 256-word steps of two
-    words per lane for MEXP 11213 / MTGP64-23209): every kind with counts
-    around the step size, unaligned outputs, a start window from partial
-    windows, a segment chain with numbers stepped over, and the queue with
-    fewer workgroups than items."""
+    words per lane for MEXP 11213 / MTGP64-23209) and layout 3 (four waves,
+    1024-word steps): every kind with counts around the step size,
+    unaligned outputs, a start window from partial windows, a segment chain
+    with numbers stepped over, and the queue with fewer workgroups than
+    items."""
     p = oracle.params(bits, mexp, 3)
-    assert p.n - p.pos >= (256 if p.n <= 363 else 512)
+    assert p.n - p.pos >= (1024 if layout == 3 else 256 if p.n <= 363 else 512)
     eb = bits // 8
     wd = np.uint32 if bits == 32 else np.uint64
     w0 = oracle.init_state(p, 99)
@@ -333,7 +334,7 @@ def test_two_wave_cells(emu, oracle, bits, mexp):
     cout = np.zeros_like(w0)
     items.append(dict(state_in=parts.ctypes.data, state_out=cout.ctypes.data, param=pyemu.set_id(bits, mexp, 3),
                       segs=segs.ctypes.data, nsegs=len(plan), parts=3, part_bytes=p.n * eb, n=p.n))
-    _run(emu, bits, items, 2, grid=3)
+    _run(emu, bits, items, layout, grid=3)
     for (raw, base, sin, sout, n), (e, we) in zip(keep, want):
         start = base - raw.ctypes.data
         got = raw[start:start + n * eb].view(e.dtype)
