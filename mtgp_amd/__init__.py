@@ -543,6 +543,9 @@ class Plan:
         _check(_lib.mtgp_plan_commit(self.h), "plan_commit")
 
     def configure(self, layout=-1, blocks_per_cu=-1):
+        """Step layout (0 quad, 1 lane-strided, 2 two-wave cells, 3 four-wave
+        cells; launches that do not admit 2 or 3 take the next narrower one)
+        and resident workgroups per CU; -1 leaves a knob unchanged."""
         _check(_lib.mtgp_plan_configure(self.h, layout, blocks_per_cu), "plan_configure")
 
     def set_overlap(self, on=True):
