@@ -191,8 +191,9 @@ void mtgp_plan_destroy(mtgp_plan_t *plan);
  * steps of two words per lane for MTGP32-11213 and MTGP64-23209; for
  * launches whose sets all have N - pos of at least that step and whose
  * start states come from the batched jump launch; 3 = four waves per cell,
- * 1024-word steps, for MTGP32-44497 and MTGP64-110503 -- launches that do
- * not qualify take the next narrower layout) and resident workgroups per CU; -1 leaves a knob
+ * 1024-word steps for MTGP32-44497 and MTGP64-110503, 512-word steps of two
+ * words per lane for MTGP32-23209 and MTGP64-44497 -- launches that do not
+ * qualify take the next narrower layout) and resident workgroups per CU; -1 leaves a knob
  * unchanged. */
 int mtgp_plan_configure(mtgp_plan_t *plan, int layout, int blocks_per_cu);
 /* Two-phase launch order (off by default; MTGP_OVERLAP=1 changes the

@@ -1209,19 +1209,21 @@ template <int B, int NMAX, int LAYOUT, int WPL = 4>
 struct KernelOf;
 // Layout 2: lane-strided two-wave cells, 512-word steps (256-word steps of
 // two words per lane for the N <= 351/363 classes, whose N - pos can be as
-// small as 258).  Layout 3: four-wave cells, 1024-word steps (the N <=
-// 1391/1727 classes).
+// small as 258).  Layout 3: four-wave cells, 1024-word steps (512-word
+// steps of two words per lane for the N <= 726/696 classes).
 template <int NMAX, int LAYOUT>
 struct KernelOf<32, NMAX, LAYOUT, 4> {
     static constexpr int waves = LAYOUT == 3 ? 4 : LAYOUT == 2 ? 2 : 1;
     static constexpr auto fn =
-        mtgp32_items_kernel<NMAX, kK, LAYOUT >= 2 ? 1 : LAYOUT, waves, LAYOUT == 2 && NMAX <= 351 ? 2 : 4>;
+        mtgp32_items_kernel<NMAX, kK, LAYOUT >= 2 ? 1 : LAYOUT, waves,
+                            (LAYOUT == 2 && NMAX <= 351) || (LAYOUT == 3 && NMAX <= 726) ? 2 : 4>;
 };
 template <int NMAX, int LAYOUT, int WPL>
 struct KernelOf<64, NMAX, LAYOUT, WPL> {
     static constexpr int waves = LAYOUT == 3 ? 4 : LAYOUT == 2 ? 2 : 1;
     static constexpr auto fn =
-        mtgp64_items_kernel<NMAX, kK, LAYOUT >= 2 ? 1 : LAYOUT, LAYOUT == 2 && NMAX <= 363 ? 2 : WPL, waves>;
+        mtgp64_items_kernel<NMAX, kK, LAYOUT >= 2 ? 1 : LAYOUT,
+                            (LAYOUT == 2 && NMAX <= 363) || (LAYOUT == 3 && NMAX <= 696) ? 2 : WPL, waves>;
 };
 
 template <class KO>
@@ -1244,10 +1246,10 @@ auto with_kernel(int bits, int cls, int layout, F &&f) {
 #define N_OF(B, c) ((B) == 32 ? ((c) == 0 ? 351 : (c) == 1 ? 726 : 1391) : ((c) == 0 ? 363 : (c) == 1 ? 696 : 1727))
     MTGP_CASE(32, 351, 0) MTGP_CASE(32, 351, 1) MTGP_CASE(32, 726, 0) MTGP_CASE(32, 726, 1)
     MTGP_CASE(32, 1391, 0) MTGP_CASE(32, 1391, 1) MTGP_CASE(32, 351, 2) MTGP_CASE(32, 726, 2)
-    MTGP_CASE(32, 1391, 2) MTGP_CASE(32, 1391, 3) MTGP_CASE(64, 363, 0) MTGP_CASE(64, 363, 1)
+    MTGP_CASE(32, 1391, 2) MTGP_CASE(32, 726, 3) MTGP_CASE(32, 1391, 3) MTGP_CASE(64, 363, 0) MTGP_CASE(64, 363, 1)
     MTGP_CASE(64, 363, 2)
     MTGP_CASE(64, 696, 0) MTGP_CASE(64, 696, 1) MTGP_CASE(64, 1727, 0) MTGP_CASE(64, 1727, 1)
-    MTGP_CASE(64, 696, 2) MTGP_CASE(64, 1727, 2) MTGP_CASE(64, 1727, 3)
+    MTGP_CASE(64, 696, 2) MTGP_CASE(64, 1727, 2) MTGP_CASE(64, 696, 3) MTGP_CASE(64, 1727, 3)
 #undef MTGP_CASE
 #undef N_OF
     return f.template operator()<KernelOf<64, 1727, 1>>();
@@ -1286,7 +1288,7 @@ bool layout_supported(int bits, int cls, int layout) {
     if (layout == 2)
         return cls <= 2;
     if (layout == 3)
-        return cls == 2;
+        return cls == 1 || cls == 2;
     return true;
 }
 
@@ -1294,7 +1296,7 @@ int layout_step_words(int bits, int cls, int layout) {
     if (bits == 64 && cls == 3)
         return 128;
     if (layout == 3)
-        return 1024;
+        return cls == 1 ? 512 : 1024;
     if (layout == 2)
         return cls == 0 ? 256 : 512;
     return 256;

@@ -18,7 +18,7 @@ constexpr int kClasses = 4;  // classes 0..2 of both word sizes, 3 for 64-bit
 // LDS, 256-byte coalesced dword stores); 2 = layout 1 with two waves per
 // cell, thread l computing words l + 128i of 512-word steps (256-word steps
 // of two words per lane for class 0); 3 = four waves per cell, 1024-word
-// steps (class 2).  Layouts 2 and 3 need N - pos of at least the step and
+// steps (class 2; 512-word steps of two words per lane for class 1).  Layouts 2 and 3 need N - pos of at least the step and
 // batched start states.
 constexpr int kLayouts = 4;
 // Whether (bits, cls) has a kernel of that layout, and its words per step.

@@ -291,7 +291,8 @@ def test_narrow_class_items_and_jumps(emu, oracle, idx):
 
 
 @pytest.mark.parametrize("bits,mexp,layout", [(32, 11213, 2), (32, 23209, 2), (32, 44497, 2), (64, 23209, 2),
-                                              (64, 44497, 2), (64, 110503, 2), (32, 44497, 3), (64, 110503, 3)])
+                                              (64, 44497, 2), (64, 110503, 2), (32, 23209, 3), (32, 44497, 3),
+                                              (64, 44497, 3), (64, 110503, 3)])
 def test_multi_wave_cells(emu, oracle, bits, mexp, layout):
     """Layout 2 (two waves per cell; 512-word steps, 256-word steps of two
     words per lane for MEXP 11213 / MTGP64-23209) and layout 3 (four waves,
@@ -300,7 +301,8 @@ def test_multi_wave_cells(emu, oracle, bits, mexp, layout):
     with numbers stepped over, and the queue with fewer workgroups than
     items."""
     p = oracle.params(bits, mexp, 3)
-    assert p.n - p.pos >= (1024 if layout == 3 else 256 if p.n <= 363 else 512)
+    step = {2: 256 if p.n <= 363 else 512, 3: 512 if p.n <= 726 else 1024}[layout]
+    assert p.n - p.pos >= step
     eb = bits // 8
     wd = np.uint32 if bits == 32 else np.uint64
     w0 = oracle.init_state(p, 99)
