@@ -427,13 +427,14 @@ __device__ __forceinline__ int segment32(uint32_t *buf, const uint32_t *tab, P32
 }
 
 // W waves per workgroup, WPL words per lane and step (W > 1: lane-strided
-// only, every set of the class has N - pos >= 64*W*WPL, and no Item carries
-// a fused jump polynomial; the planner guarantees all three).
+// only, every Item's set has N - pos >= 64*W*WPL, and no Item carries a
+// fused jump polynomial; the planner guarantees all three).  The fused
+// jump steps 256 words at a time, so single-wave kernels keep WPL = 4.
 template <int NMAX, int K, int LAYOUT, int W = 1, int WPL = 4>
 __global__ __launch_bounds__(64 * W) void mtgp32_items_kernel(const Item *__restrict__ items, int n_items,
                                                               const DevParam *__restrict__ params,
                                                               uint32_t *__restrict__ queue) {
-    static_assert(W == 1 || WPL == 4 || WPL == 2, "words per lane");
+    static_assert(WPL == 4 || (W > 1 && WPL == 2), "words per lane");
     constexpr int T = kWave * W;
     constexpr int NJ = (NMAX + T - 1) / T;
     constexpr int NBUF = K * T * WPL + T * NJ + T;
