@@ -453,7 +453,7 @@ def main():
                    "serial_ms_per_step": round(r["serial_ms"] / args.steps, 4),
                    "jump_ms": round(sum(r["jump_ms"]) / len(r["jump_ms"]), 4),
                    "generate_ms": round(gen_avg, 4)},
-        "roofline": {"bound": "hbm", "kernel": "mtgp32_items_kernel<351,3,1>", "achieved": round(kern_gbs, 2),
+        "roofline": {"bound": "hbm", "kernel": "mtgp32_items_kernel<351,3,1,1,4>", "achieved": round(kern_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(kern_gbs / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes": alg_bytes, "traffic": measured_traffic()},
     }
