@@ -24,6 +24,8 @@ constexpr int kLayouts = 4;
 // Whether (bits, cls) has a kernel of that layout, and its words per step.
 bool layout_supported(int bits, int cls, int layout);
 int layout_step_words(int bits, int cls, int layout);
+// Waves per workgroup (per cell) of a layout.
+inline int layout_waves(int layout) { return layout == 3 ? 4 : layout == 2 ? 2 : 1; }
 // Kernel entry (for occupancy queries) and its workgroup size.
 const void *kernel_fn(int bits, int cls, int layout);
 int kernel_threads(int bits, int cls, int layout);

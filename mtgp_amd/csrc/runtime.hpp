@@ -181,8 +181,11 @@ GridModel grid_model_of(int bits, int cls, int mexp, int n);
 // numbers-weighted mean jump_cost_numbers of the plan's generators: when a
 // jump costs much less than a cell of one round over every slot, the
 // two-size grid (two rounds); otherwise one size, with fewer cells the
-// dearer a jump is.
-void auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_gens, int slots, double jump_cost,
+// dearer a jump is.  `slots` counts resident workgroups; the multi-round
+// grids size their rounds in waves (slots / model.cell_waves cells), so
+// multi-wave cells keep the waves per CU of single-wave ones.  Returns true
+// for the one-round grid of the step-time model.
+bool auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_gens, int slots, double jump_cost,
                const GridModel &model, uint64_t &S1, uint64_t &S2, double &frac);
 
 // Jump polynomials x^pos mod P kept per parameter slot for positions on a
@@ -284,6 +287,9 @@ struct Plan {
     int jumps = 0;         // cells that start with a jump
     double plan_ms = 0;
     bool has_phase1 = false;  // the small cells form launches of their own
+    // Multi-round grid (or caller-sized cells): default workgroups per CU
+    // of multi-wave layouts divided by their waves.
+    bool waves_scaled = false;
     bool overlap = false;
     // Batched jumps are split into partial jumps (chunk ranges, XORed when
     // the generation kernel loads the start window) until a jump launch has
