@@ -148,7 +148,7 @@ Context &context() {
             for (int b = 0; b < 2; b++)
                 for (int cls = 0; cls < 3; cls++)  // (class 3 is lane-strided only)
                     if (layout_supported(b ? 64 : 32, cls, lay))
-                        c->layout[b][cls] = lay;
+                        c->layout[b][cls] = c->layout_one_round[b][cls] = lay;
     }
     Context &ref = *c;
     g_ctx[dev] = std::move(c);
@@ -245,8 +245,8 @@ int Context::layout_within(int lay, int mask) {
     return lay;
 }
 
-int Context::layout_for(int bits, int cls, int mask) const {
-    return layout_within(layout[bits == 64][cls], mask);
+int Context::layout_for(int bits, int cls, int mask, bool one_round) const {
+    return layout_within(one_round ? layout_one_round[bits == 64][cls] : layout[bits == 64][cls], mask);
 }
 
 const DevParam &Context::param(int slot) const {
@@ -788,7 +788,7 @@ bool auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_g
     // Multi-round grids: rounds of slots / cell_waves cells (a large
     // MTGP64-23209 batch in two-wave cells: 4.88 TB/s at 6 per CU against
     // 4.54 at 12 and 4.55 in single-wave cells at 12).
-    const int wslots = std::max(1, (int)std::lround((double)slots / std::max(1.0, model.cell_waves)));
+    const int wslots = std::max(1, (int)std::lround((double)slots / std::max(1.0, model.cell_waves_multi)));
     const double wusable = std::max(1.0, (double)wslots - (double)n_requests);
     frac = c.segment_split;
     if (frac <= 0 || frac >= 1) {
@@ -855,11 +855,13 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     GridModel model;
     model.rate = 0;
     model.cell_waves = 0;
+    model.cell_waves_multi = 0;
     double w0 = 0;
     for (int g : named) {
         const GenRef &gr = gens[(size_t)g];
         const int b = gr.bits == 64, cls = c.class_of(gr.slot);
         const int lay = c.layout_for(gr.bits, cls, group_ok[3 * b + cls]);
+        const int lay1 = c.layout_for(gr.bits, cls, group_ok[3 * b + cls], true);
         slots = std::max(slots, c.cus * std::min(c.bpc[b][cls], c.blocks_per_cu[b][cls][lay]));
         const double nw = (double)per[(size_t)g];
         jcost += nw * jump_cost_numbers(gr.mexp, gr.n);
@@ -867,7 +869,8 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         model.jump_ns += nw * gm.jump_ns;
         model.rate += nw * gm.rate;
         model.bytes += nw * (gr.bits / 8);
-        model.cell_waves += nw * (lay == 3 ? 4 : lay == 2 ? 2 : 1);
+        model.cell_waves += nw * layout_waves(lay1);
+        model.cell_waves_multi += nw * layout_waves(lay);
         w0 += nw * gm.w0;
         weight += nw;
         active += per[(size_t)g] > 0;
@@ -878,9 +881,10 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         model.rate /= weight;
         model.w0 = w0 / weight;
         model.cell_waves /= weight;
+        model.cell_waves_multi /= weight;
     } else {
         model.rate = 4700;
-        model.cell_waves = 1;
+        model.cell_waves = model.cell_waves_multi = 1;
     }
     uint64_t S1 = 0, S2 = 0;
     double frac = 0;
@@ -895,6 +899,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     else
         one_round = auto_grid(c, all, nonempty, active, slots, jcost, model, S1, S2, frac);
     waves_scaled = !one_round;
+    one_round_grid = one_round;
     const uint64_t S = S1;
     // Requests as runs of stream positions per generator.
     struct Run {
@@ -1291,7 +1296,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
                 L.grid = std::min(L.count, c.cus * c.jump_blocks_per_cu[L.bits == 64][L.cls]);
                 L.dyn_lds = 0;
             } else {
-                L.layout = c.layout_for(L.bits, L.cls, L.layouts_ok);
+                L.layout = c.layout_for(L.bits, L.cls, L.layouts_ok, one_round);
                 const int cap = c.blocks_per_cu[L.bits == 64][L.cls][L.layout];
                 int bpc = std::min(c.bpc[L.bits == 64][L.cls], cap);
                 if (waves_scaled)
@@ -1502,7 +1507,7 @@ float Plan::autotune(hipStream_t s, int reps, int *best_layout, int *best_bpc) {
     configure(bl, bb);
     for (const auto &L : launches) {
         if (L.layout == bl)
-            ctx->layout[L.bits == 64][L.cls] = bl;
+            (one_round_grid ? ctx->layout_one_round : ctx->layout)[L.bits == 64][L.cls] = bl;
         ctx->bpc[L.bits == 64][L.cls] = bb;
     }
     if (best_layout)

@@ -46,6 +46,11 @@ struct Context {
     // single-wave cells: two-wave ones measured 4.16 / 4.25 / 4.04 TB/s at 6
     // / 8 / 12 per CU against 4.21.
     int layout[2][4] = {{1, 2, 3, 1}, {2, 2, 2, 1}};
+    // The same for the step-time model's one-round grids, where MTGP32-11213
+    // also gains from two-wave cells (one stream of 2^24 + 2^24 numbers:
+    // 0.500 against 0.467 TB/s per step, generation 0.074 against 0.093 ms)
+    // while several rounds of them lose (4096 substreams: 4.21 against 4.46).
+    int layout_one_round[2][4] = {{2, 2, 3, 1}, {2, 2, 2, 1}};
     // Resident workgroups per CU used by default.  The generation kernel is
     // held back by the board's power budget (its clock drops to 1.6-2.0 GHz
     // under full store traffic), so more waves do not generate faster but
@@ -123,8 +128,9 @@ struct Context {
     // `lay` where the mask allows it, else the next narrower layout (3 -> 2
     // -> 1).
     static int layout_within(int lay, int mask);
-    // The default layout of (bits, class) within the mask.
-    int layout_for(int bits, int cls, int mask) const;
+    // The default layout of (bits, class) within the mask, for a one-round
+    // grid or any other.
+    int layout_for(int bits, int cls, int mask, bool one_round = false) const;
     void sha1_of(int slot, unsigned char out[20]) const;
 };
 
@@ -170,7 +176,8 @@ struct GridModel {
     double jump_ns = 0;   // chip time of one batched jump
     double rate = 4700;   // saturated generation rate, GB/s
     double w0 = 5.0;      // waves per CU at which the rate is 63% of it
-    double cell_waves = 1;  // waves per cell (2 for two-wave cells)
+    double cell_waves = 1;        // waves per cell on the one-round grid
+    double cell_waves_multi = 1;  // the same on multi-round grids
 };
 // Measured on MI355X per (word size, state class); classes without a
 // measurement use jump_ns = 4.5e-5 * MEXP * N * (bits / 32) / 2 (the jump's
@@ -290,6 +297,7 @@ struct Plan {
     // Multi-round grid (or caller-sized cells): default workgroups per CU
     // of multi-wave layouts divided by their waves.
     bool waves_scaled = false;
+    bool one_round_grid = false;  // cells from the step-time model, one round
     bool overlap = false;
     // Batched jumps are split into partial jumps (chunk ranges, XORed when
     // the generation kernel loads the start window) until a jump launch has
