@@ -766,9 +766,13 @@ bool auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_g
         // CU: an uneven count leaves a CU, or a SIMD, one wave heavier, and
         // its waves finish last (measured: MTGP64-44497 generates no faster
         // at 5 per CU than at 4).
+        // With four-wave cells, w = 1..3 (4 to 12 waves per CU) are
+        // candidates too (MTGP64-44497: 512 four-wave cells 3.82 TB/s
+        // against 3.46 for 1024).
         const int wmax = std::max(1, slots / std::max(1, c.cus));
+        const int wstep = model.cell_waves >= 4 ? 1 : 4;
         double best = 0, best_t = 0;
-        for (int w = std::min(4, wmax); w <= wmax; w += 4) {
+        for (int w = std::min(wstep, wmax); w <= wmax; w += wstep) {
             const double jumps = std::max(0.0, (double)w * c.cus - (double)n_gens);
             const double t_jump = c.jump_floor_ms * 1e6 + jumps * model.jump_ns;
             const double waves = (double)w * model.cell_waves;
@@ -1298,9 +1302,10 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             } else {
                 L.layout = c.layout_for(L.bits, L.cls, L.layouts_ok, one_round);
                 const int cap = c.blocks_per_cu[L.bits == 64][L.cls][L.layout];
-                int bpc = std::min(c.bpc[L.bits == 64][L.cls], cap);
-                if (waves_scaled)
-                    bpc = std::max(1, bpc / layout_waves(L.layout));
+                // (the knob counts waves there; the occupancy cap counts
+                // workgroups)
+                const int knob = c.bpc[L.bits == 64][L.cls];
+                int bpc = std::min(waves_scaled ? std::max(1, knob / layout_waves(L.layout)) : knob, cap);
                 // Up to half a round more cells than slots (e.g. one cell per
                 // generator for thousands of generators) run as one round
                 // when the occupancy limit allows: a partial second round

@@ -38,19 +38,20 @@ struct Context {
     int blocks_per_cu[2][4][4] = {};  // [bits == 64][class][layout]
     int jump_blocks_per_cu[2][4] = {};
     // Multi-wave cells by default where they measured faster: on 64-set
-    // batches of MTGP32-23209 / MTGP64-23209 / MTGP64-44497, two-wave cells
-    // (layout 2) 3.32 / 4.09 / 3.42 TB/s per step against 2.60 / 3.86-3.95 /
-    // 2.83 with single-wave cells; MTGP32-44497, four-wave cells (layout 3)
-    // 2.86 against 2.20 (two-wave) and 1.78 (more waves per CU from fewer
-    // cells, so fewer start-state jumps).  Batched MTGP32-11213 keeps
+    // batches of MTGP32-23209 / MTGP64-23209, two-wave cells (layout 2) 3.32
+    // / 4.09 TB/s per step against 2.60 / 3.86-3.95 with single-wave cells;
+    // MTGP32-44497 / MTGP64-44497, four-wave cells (layout 3) on 768 / 512
+    // cells 3.03 / 3.76 against 2.22 / 3.49 with two-wave cells and 1.78 /
+    // 2.83 with single-wave ones (more waves per CU from fewer cells, so
+    // fewer start-state jumps).  Batched MTGP32-11213 keeps
     // single-wave cells: two-wave ones measured 4.16 / 4.25 / 4.04 TB/s at 6
     // / 8 / 12 per CU against 4.21.
-    int layout[2][4] = {{1, 2, 3, 1}, {2, 2, 2, 1}};
+    int layout[2][4] = {{1, 2, 3, 1}, {2, 3, 2, 1}};
     // The same for the step-time model's one-round grids, where MTGP32-11213
     // also gains from two-wave cells (one stream of 2^24 + 2^24 numbers:
     // 0.500 against 0.467 TB/s per step, generation 0.074 against 0.093 ms)
     // while several rounds of them lose (4096 substreams: 4.21 against 4.46).
-    int layout_one_round[2][4] = {{2, 2, 3, 1}, {2, 2, 2, 1}};
+    int layout_one_round[2][4] = {{2, 2, 3, 1}, {2, 3, 2, 1}};
     // Resident workgroups per CU used by default.  The generation kernel is
     // held back by the board's power budget (its clock drops to 1.6-2.0 GHz
     // under full store traffic), so more waves do not generate faster but
