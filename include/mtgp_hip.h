@@ -178,12 +178,25 @@ typedef struct mtgp_request {
  * and a persistent generation launch on `stream`; it can be called again
  * (the same numbers are produced again, from the same states) and captured
  * in a hipGraph.  mtgp_plan_commit makes the states after the plan the
- * generators' current states. */
+ * generators' current states; it fails (-1, nothing changes) if the plan
+ * was not launched since it was built or rebased, or if a generator's state
+ * has changed since then (another plan's commit, mtgp_genset_jump,
+ * set_state, init or substreams), since committing would rewind it.  A
+ * second commit of the same launch does nothing. */
 typedef struct mtgp_plan mtgp_plan_t;
 mtgp_plan_t *mtgp_plan_create(mtgp_genset_t *set, const mtgp_request_t *reqs, int nreq,
                               uint64_t segment);
 int mtgp_plan_launch(mtgp_plan_t *plan, void *stream);
 int mtgp_plan_commit(mtgp_plan_t *plan);
+/* Re-targets a plan at its generators' current states (the same requests,
+ * outputs, cells and jump polynomials): after a commit, the next launch
+ * continues every stream with the next numbers, the way the reference's
+ * device samples continue from the state they wrote back
+ * (cuda-sample/mtgp32-cuda.cu:151-161).  Host work is patching the work
+ * items' state addresses and one small asynchronous upload on `stream`,
+ * which must be the stream the plan launches on (or ordered after its last
+ * launch); no polynomial work.  A captured graph stays valid. */
+int mtgp_plan_rebase(mtgp_plan_t *plan, void *stream);
 void mtgp_plan_destroy(mtgp_plan_t *plan);
 /* Tuning knobs (the autotuner's): step layout (0 = four consecutive
  * numbers per lane, 128-bit stores; 1 = lane-strided, conflict-free LDS;
@@ -226,6 +239,13 @@ int mtgp_plan_autotune(mtgp_plan_t *plan, void *stream, int reps, int *layout, i
 /* Work items, kernel launches of one mtgp_plan_launch (jump and
  * generation), and host planning time of a plan. */
 int mtgp_plan_stats(const mtgp_plan_t *plan, int *items, int *launches, double *plan_ms);
+/* Generation launch k of mtgp_plan_launch_part(plan, 1, ...), in order:
+ * word size, state-size class, step layout, workgroups, work items and the
+ * kernel's template name as profilers print it (written to kernel[cap] when
+ * kernel is not NULL).  Returns the number of such launches (k < 0: only
+ * that), or -1. */
+int mtgp_plan_launch_info(const mtgp_plan_t *plan, int k, int *bits, int *cls, int *layout, int *grid,
+                          int *items, char *kernel, size_t cap);
 /* plan + launch + commit + synchronize. */
 int mtgp_genset_generate(mtgp_genset_t *set, const mtgp_request_t *reqs, int nreq, void *stream);
 

@@ -138,6 +138,9 @@ _sig("mtgp_genset_substreams", i32, vp, vp, cp)
 _sig("mtgp_plan_create", vp, vp, C.POINTER(Request), i32, u64)
 _sig("mtgp_plan_launch", i32, vp, vp)
 _sig("mtgp_plan_commit", i32, vp)
+_sig("mtgp_plan_rebase", i32, vp, vp)
+_sig("mtgp_plan_launch_info", i32, vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+     C.POINTER(C.c_int), C.POINTER(C.c_int), cp, sz)
 _sig("mtgp_plan_destroy", None, vp)
 _sig("mtgp_plan_configure", i32, vp, i32, i32)
 _sig("mtgp_plan_set_overlap", i32, vp, i32)
@@ -206,7 +209,10 @@ def _wdtype(bits):
 
 
 def out_dtype(bits, kind):
-    if kind in (KIND_INT, KIND_CLOSE1_OPEN2):
+    """Element type of `kind`: unsigned words for genrand_uint*, float32 /
+    float64 for the four floating-point kinds (close1_open2 included, as
+    mtgp*_genrand_close1_open2 returns a float / double)."""
+    if kind == KIND_INT:
         return np.uint32 if bits == 32 else np.uint64
     return np.float32 if bits == 32 else np.float64
 
@@ -391,7 +397,10 @@ class _Generator:
         if isinstance(out, DeviceBuffer) or hasattr(out, "ptr"):
             ptr = out.ptr
         else:
-            if out.dtype != dt or out.size < n or not out.flags.c_contiguous:
+            # a float kind may also be written as its raw bits (an unsigned
+            # array of the same width)
+            ok = out.dtype == dt or out.dtype == _wdtype(self.bits)
+            if not ok or out.size < n or not out.flags.c_contiguous:
                 raise ValueError("output array must be contiguous %s of >= %d" % (dt, n))
             ptr = out.ctypes.data
         _check(getattr(_lib, "mtgp%d_fill" % self.bits)(C.byref(self.g), kind, ptr, n), "fill")
@@ -542,6 +551,11 @@ class Plan:
     def commit(self):
         _check(_lib.mtgp_plan_commit(self.h), "plan_commit")
 
+    def rebase(self, stream=None):
+        """Points the plan at the generators' current states: the next launch
+        writes the next numbers of every stream into the same outputs."""
+        _check(_lib.mtgp_plan_rebase(self.h, stream.handle if stream else None), "plan_rebase")
+
     def configure(self, layout=-1, blocks_per_cu=-1):
         """Step layout (0 quad, 1 lane-strided, 2 two-wave cells, 3 four-wave
         cells; launches that do not admit 2 or 3 take the next narrower one)
@@ -569,6 +583,20 @@ class Plan:
         _check(_lib.mtgp_plan_autotune(self.h, stream.handle if stream else None, reps, C.byref(lay), C.byref(bpc),
                                        C.byref(ms)), "plan_autotune")
         return lay.value, bpc.value, ms.value
+
+    def launches(self):
+        """The generation launches in serial order: dicts of bits, class,
+        layout, grid, items and the kernel's template name."""
+        n = _check(_lib.mtgp_plan_launch_info(self.h, -1, None, None, None, None, None, None, 0), "plan_launch_info")
+        out = []
+        for k in range(n):
+            v = [C.c_int() for _ in range(5)]
+            name = C.create_string_buffer(128)
+            _check(_lib.mtgp_plan_launch_info(self.h, k, *[C.byref(x) for x in v], name, len(name)),
+                   "plan_launch_info")
+            out.append(dict(zip(("bits", "cls", "layout", "grid", "items"), [x.value for x in v]),
+                            kernel=name.value.decode()))
+        return out
 
     def stats(self):
         it, ln, ms = C.c_int(), C.c_int(), C.c_double()

@@ -56,7 +56,9 @@ constexpr int kJumpSteps = 4;                  // 256-word steps per chunk
 constexpr int kJumpChunk = kJumpSteps * 256;   // positions per chunk
 // Chunks [c0, c1) are folded; the sequence is generated from chunk 0.  A
 // jump split into parts writes partial windows whose XOR is the jumped
-// window (Item.parts).
+// window (Item.parts).  The base window may itself be such a split result:
+// with in_parts > 1 it is the XOR of in_parts windows in_part_bytes apart
+// at state_in.
 struct alignas(16) JumpItem {
     uint64_t state_in;
     uint64_t masks;       // const uint64_t *: [chunk][64 residues]
@@ -65,8 +67,9 @@ struct alignas(16) JumpItem {
     int32_t c0, c1;
     int32_t param;
     int32_t state_bytes;
-    int32_t pad0;
-    uint64_t pad[2];
+    int32_t in_parts;     // 0 or 1: one window at state_in
+    uint64_t in_part_bytes;
+    uint64_t pad;
 };
 static_assert(sizeof(JumpItem) == 64, "JumpItem layout");
 

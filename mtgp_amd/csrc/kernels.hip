@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "common.hpp"
 #include "kernels.hpp"
 // Angle brackets: the kernel emulation build substitutes its own copy.
@@ -64,12 +66,16 @@ constexpr int kTabWords = 3 * kTabRep;
 
 __device__ __forceinline__ uint32_t lookup(const uint32_t *t, uint32_t v) { return t[v & 15]; }
 
-// Fills tab[kTabWords] from a parameter record (DevParam's tbl, tmp and
-// flt are contiguous).
+// Fills tab[kTabWords] from a parameter record: tbl, tmp and flt, each
+// read through its own array (one table per group of 16 lanes).
 __device__ __forceinline__ void fill_tables(uint32_t *tab, const DevParam &D) {
     const int l = threadIdx.x;
-    if (l < kTabWords)
+    if (l < kTabRep)
         tab[l] = D.tbl[l];
+    else if (l < 2 * kTabRep)
+        tab[l] = D.tmp[l - kTabRep];
+    else if (l < kTabWords)
+        tab[l] = D.flt[l - 2 * kTabRep];
 }
 
 // t ^= t >> 16; t ^= t >> 8 (mtgp32-fast.h:196-197); the low four bits
@@ -1052,9 +1058,15 @@ __global__ __launch_bounds__(64) void mtgp32_jump_kernel(const JumpItem *__restr
         fill_tables(tab, D);
         const gu32 *in = reinterpret_cast<const gu32 *>(I.state_in);
         const gu64c *mk = reinterpret_cast<const gu64c *>(I.masks);
+        const int in_parts = I.in_parts;
+        const uint64_t in_stride = I.in_part_bytes / 4;
         wave_sync();
-        for (int j = l; j < p.n; j += kWave)
-            buf[j] = in[j];
+        for (int j = l; j < p.n; j += kWave) {
+            uint32_t v = in[j];
+            for (int q = 1; q < in_parts; q++)
+                v ^= in[(uint64_t)q * in_stride + (uint64_t)j];
+            buf[j] = v;
+        }
         wave_sync();
         uint32_t acc[B][NJ];
 #pragma unroll
@@ -1136,9 +1148,13 @@ __global__ __launch_bounds__(64) void mtgp64_jump_kernel(const JumpItem *__restr
         fill_tables(tab, D);
         const gu64 *in = reinterpret_cast<const gu64 *>(I.state_in);
         const gu64c *mk = reinterpret_cast<const gu64c *>(I.masks);
+        const int in_parts = I.in_parts;
+        const uint64_t in_stride = I.in_part_bytes / 8;
         wave_sync();
         for (int j = l; j < p.n; j += kWave) {
-            const uint64_t v = in[j];
+            uint64_t v = in[j];
+            for (int q = 1; q < in_parts; q++)
+                v ^= in[(uint64_t)q * in_stride + (uint64_t)j];
             bh[j] = (uint32_t)(v >> 32);
             bl[j] = (uint32_t)v;
         }
@@ -1215,16 +1231,24 @@ struct KernelOf;
 template <int NMAX, int LAYOUT>
 struct KernelOf<32, NMAX, LAYOUT, 4> {
     static constexpr int waves = LAYOUT == 3 ? 4 : LAYOUT == 2 ? 2 : 1;
-    static constexpr auto fn =
-        mtgp32_items_kernel<NMAX, kK, LAYOUT >= 2 ? 1 : LAYOUT, waves,
-                            (LAYOUT == 2 && NMAX <= 351) || (LAYOUT == 3 && NMAX <= 726) ? 2 : 4>;
+    static constexpr int step_layout = LAYOUT >= 2 ? 1 : LAYOUT;
+    static constexpr int wpl = (LAYOUT == 2 && NMAX <= 351) || (LAYOUT == 3 && NMAX <= 726) ? 2 : 4;
+    static constexpr auto fn = mtgp32_items_kernel<NMAX, kK, step_layout, waves, wpl>;
+    static std::string name() {
+        return "mtgp32_items_kernel<" + std::to_string(NMAX) + "," + std::to_string(kK) + "," +
+               std::to_string(step_layout) + "," + std::to_string(waves) + "," + std::to_string(wpl) + ">";
+    }
 };
 template <int NMAX, int LAYOUT, int WPL>
 struct KernelOf<64, NMAX, LAYOUT, WPL> {
     static constexpr int waves = LAYOUT == 3 ? 4 : LAYOUT == 2 ? 2 : 1;
-    static constexpr auto fn =
-        mtgp64_items_kernel<NMAX, kK, LAYOUT >= 2 ? 1 : LAYOUT,
-                            (LAYOUT == 2 && NMAX <= 363) || (LAYOUT == 3 && NMAX <= 696) ? 2 : WPL, waves>;
+    static constexpr int step_layout = LAYOUT >= 2 ? 1 : LAYOUT;
+    static constexpr int wpl = (LAYOUT == 2 && NMAX <= 363) || (LAYOUT == 3 && NMAX <= 696) ? 2 : WPL;
+    static constexpr auto fn = mtgp64_items_kernel<NMAX, kK, step_layout, wpl, waves>;
+    static std::string name() {
+        return "mtgp64_items_kernel<" + std::to_string(NMAX) + "," + std::to_string(kK) + "," +
+               std::to_string(step_layout) + "," + std::to_string(wpl) + "," + std::to_string(waves) + ">";
+    }
 };
 
 template <class KO>
@@ -1279,6 +1303,10 @@ const void *kernel_fn(int bits, int cls, int layout) {
 
 int kernel_threads(int bits, int cls, int layout) {
     return with_kernel(bits, cls, layout, []<class KO>() { return kWave * KO::waves; });
+}
+
+std::string kernel_name(int bits, int cls, int layout) {
+    return with_kernel(bits, cls, layout, []<class KO>() { return KO::name(); });
 }
 
 bool layout_supported(int bits, int cls, int layout) {

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime_api.h>
 #include "common.hpp"
 
+#include <string>
+
 namespace mtgp {
 
 // Kernel class for a state of n words (-1 if none fits), assuming
@@ -29,6 +31,8 @@ inline int layout_waves(int layout) { return layout == 3 ? 4 : layout == 2 ? 2 :
 // Kernel entry (for occupancy queries) and its workgroup size.
 const void *kernel_fn(int bits, int cls, int layout);
 int kernel_threads(int bits, int cls, int layout);
+// Its template name, as profilers print it (e.g. mtgp32_items_kernel<351,3,1,1,4>).
+std::string kernel_name(int bits, int cls, int layout);
 // Runs items[0, n_items) (device array, all of word size `bits` and class
 // <= cls) with `grid` persistent workgroups.  `queue` is a device pair of
 // uint32_t counters that is zero on entry and zero again on exit.

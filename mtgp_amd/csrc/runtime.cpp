@@ -74,6 +74,7 @@ Context &context() {
     c->arch = prop.gcnArchName;
     // Parameter table: the 768 reference sets, then room for custom sets.
     c->param_capacity = kNumSets + 1024;
+    c->customs.reserve((size_t)(c->param_capacity - kNumSets));
     hip_check(hipMalloc((void **)&c->d_params, sizeof(DevParam) * (size_t)c->param_capacity), "hipMalloc params");
     std::vector<DevParam> host(kNumSets);
     for (int k = 0; k < kNumSets; k++)
@@ -809,6 +810,9 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
                  const PlanOptions &opt) {
     const auto t0 = std::chrono::steady_clock::now();
     ctx = &c;
+    // The planner defaults (layouts, workgroups per CU, grid knobs) stay
+    // fixed while this plan reads them.
+    std::shared_lock<std::shared_mutex> tune_lock(c.tune_mu);
     const int G = (int)gens.size();
     uint64_t all = 0;
     int nonempty = 0;
@@ -1115,7 +1119,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         const int fit = (c.cus * occ) / ng;
         parts = std::max(1, std::min({want, fit, std::max(1, opt.max_parts > 0 ? opt.max_parts : c.jump_max_parts)}));
     }
-    std::vector<int> jgroup;
+    std::vector<int> jgroup, jgen_list;
     jitems.clear();
     size_t arena_bytes = 0;
     // (g, cell) -> (arena offset, parts, part stride)
@@ -1140,11 +1144,13 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             ji.state_bytes = (int32_t)sbytes;
             jitems.push_back(ji);
             jgroup.push_back(2 * group_of(ref) + gr.phase);
+            jgen_list.push_back(gr.g);
         }
         arena_bytes += align256((size_t)P * nb * sbytes);
     }
     // Items, one per cell.
     std::vector<Item> list;
+    std::vector<int> list_gen;
     std::vector<int> group;
     std::vector<uint64_t> arena_slot;  // per item: arena offset + 1 (0 = none)
     std::vector<uint64_t> cost;
@@ -1203,6 +1209,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             }
             numbers += n;
             list.push_back(it);
+            list_gen.push_back(g);
             arena_slot.push_back(slot);
             cost.push_back(n + (it.poly != UINT64_MAX ? 200000u : 0u));
             group.push_back(2 * group_of(gr) + phase_of(g, k));
@@ -1226,9 +1233,11 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         std::stable_sort(jorder.begin(), jorder.end(), [&](int a, int b) { return jgroup[(size_t)a] < jgroup[(size_t)b]; });
         std::vector<JumpItem> sorted;
         std::vector<int> sg;
+        jitem_gen.clear();
         for (int i : jorder) {
             sorted.push_back(jitems[(size_t)i]);
             sg.push_back(jgroup[(size_t)i]);
+            jitem_gen.push_back(jgen_list[(size_t)i]);
         }
         jitems.swap(sorted);
         jgroup.swap(sg);
@@ -1240,6 +1249,9 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     const size_t o_masks = align256(o_polys + poly_host.size() * 4);
     const size_t o_jitems = align256(o_masks + mask_host.size() * 8);
     const size_t work_bytes = align256(o_jitems + jitems.size() * sizeof(JumpItem));
+    this->o_items = o_items;
+    this->o_jitems = o_jitems;
+    wait_upload();
     grow_device(d_work, work_cap, work_bytes, "hipMalloc plan workspace");
     grow_pinned(h_stage, stage_cap, work_bytes);
     if (arena_bytes)
@@ -1252,6 +1264,8 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     d_items = reinterpret_cast<Item *>(d_work + o_items);
     d_jitems = reinterpret_cast<JumpItem *>(d_work + o_jitems);
     items.clear();
+    item_gen.clear();
+    item_io.clear();
     for (int i : order) {
         Item it = list[(size_t)i];
         it.poly = it.poly == UINT64_MAX ? 0 : dw + o_polys + 4 * it.poly;
@@ -1260,7 +1274,10 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         if (arena_slot[(size_t)i])
             it.state_in = (uint64_t)(uintptr_t)d_jstates + arena_slot[(size_t)i] - 1;
         items.push_back(it);
+        item_gen.push_back(list_gen[(size_t)i]);
+        item_io.push_back((uint8_t)((arena_slot[(size_t)i] ? 0 : 1) | (it.state_out ? 2 : 0)));
     }
+    all_commits = commits;
     for (auto &ji : jitems) {
         ji.masks = dw + o_masks + ji.masks;
         ji.states_out = (uint64_t)(uintptr_t)d_jstates + ji.states_out;
@@ -1276,6 +1293,9 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     if (!jitems.empty())
         memcpy(h_stage + o_jitems, jitems.data(), jitems.size() * sizeof(JumpItem));
     hip_check(hipMemcpyAsync(d_work, h_stage, work_bytes, hipMemcpyHostToDevice, opt.stream), "plan upload");
+    if (!ev_upload)
+        hip_check(hipEventCreateWithFlags(&ev_upload, hipEventDisableTiming), "hipEventCreateWithFlags");
+    hip_check(hipEventRecord(ev_upload, opt.stream), "hipEventRecord");
     if (opt.sync)
         hip_check(hipStreamSynchronize(opt.stream), "plan upload sync");
     // Launches: one jump launch and one generation launch per (word size,
@@ -1340,6 +1360,39 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     overlap = opt.overlap >= 0 ? opt.overlap != 0 : c.overlap;
     segment_used = S;
     plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void Plan::wait_upload() {
+    if (ev_upload)
+        hip_check(hipEventSynchronize(ev_upload), "hipEventSynchronize");
+}
+
+void Plan::retarget(const std::vector<GenRef> &gens, hipStream_t s) {
+    if (!d_work)
+        throw Error("plan has no work list");
+    for (size_t i = 0; i < items.size(); i++) {
+        const GenRef &gr = gens.at((size_t)item_gen[i]);
+        if (item_io[i] & 1)
+            items[i].state_in = (uint64_t)(uintptr_t)gr.state_in;
+        if (item_io[i] & 2)
+            items[i].state_out = (uint64_t)(uintptr_t)gr.state_out;
+    }
+    for (size_t i = 0; i < jitems.size(); i++)
+        jitems[i].state_in = (uint64_t)(uintptr_t)gens.at((size_t)jitem_gen[i]).state_in;
+    wait_upload();
+    const size_t ib = items.size() * sizeof(Item), jb = jitems.size() * sizeof(JumpItem);
+    if (ib) {
+        memcpy(h_stage + o_items, items.data(), ib);
+        hip_check(hipMemcpyAsync(d_work + o_items, h_stage + o_items, ib, hipMemcpyHostToDevice, s), "plan upload");
+    }
+    if (jb) {
+        memcpy(h_stage + o_jitems, jitems.data(), jb);
+        hip_check(hipMemcpyAsync(d_work + o_jitems, h_stage + o_jitems, jb, hipMemcpyHostToDevice, s), "plan upload");
+    }
+    if (!ev_upload)
+        hip_check(hipEventCreateWithFlags(&ev_upload, hipEventDisableTiming), "hipEventCreateWithFlags");
+    hip_check(hipEventRecord(ev_upload, s), "hipEventRecord");
+    commits = all_commits;
 }
 
 void Plan::reserve(size_t work_bytes, size_t jstates_bytes) {
@@ -1408,6 +1461,7 @@ void Plan::run_launches(int part, int phase, hipStream_t s) {
 
 void Plan::configure(int layout, int bpc) {
     drop_graph();
+    std::shared_lock<std::shared_mutex> tune_lock(ctx->tune_mu);
     for (auto *Ls : {&launches, &mlaunches})
     for (auto &L : *Ls) {
         if (layout >= 0) {
@@ -1510,6 +1564,7 @@ float Plan::autotune(hipStream_t s, int reps, int *best_layout, int *best_bpc) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     configure(bl, bb);
+    std::unique_lock<std::shared_mutex> tune_lock(ctx->tune_mu);
     for (const auto &L : launches) {
         if (L.layout == bl)
             (one_round_grid ? ctx->layout_one_round : ctx->layout)[L.bits == 64][L.cls] = bl;
@@ -1524,6 +1579,9 @@ float Plan::autotune(hipStream_t s, int reps, int *best_layout, int *best_bpc) {
 
 void Plan::free_device() {
     drop_graph();
+    if (ev_upload)
+        (void)hipEventDestroy(ev_upload);
+    ev_upload = nullptr;
     if (ev_fork)
         (void)hipEventDestroy(ev_fork);
     if (ev_join)
@@ -1563,9 +1621,25 @@ using namespace mtgp;
         return fail;                    \
     }
 
+// A plan over a genset remembers, per committed generator, the slot its
+// launch writes and the generator's epoch when the plan was built or
+// rebased; commit refuses a plan that was never launched or whose
+// generators' states have changed since (which would rewind them).
 struct mtgp_plan {
     mtgp_genset *set;
     Plan plan;
+    std::vector<int> out_slot;
+    std::vector<uint64_t> epochs;
+    bool launched = false;
+    void record_targets() {
+        out_slot.clear();
+        epochs.clear();
+        for (int g : plan.commits) {
+            out_slot.push_back(set->cur[(size_t)g] ^ 1);
+            epochs.push_back(set->epoch[(size_t)g]);
+        }
+        launched = false;
+    }
 };
 
 // Applies a jump polynomial (hex) to device window `in`, result in `out`,
@@ -1775,7 +1849,7 @@ int mtgp_tuning_set(const char *name, double value) {
     API_BEGIN
     Context &c = context();
     const std::string n = name ? name : "";
-    std::lock_guard<std::mutex> lk(c.mu);
+    std::unique_lock<std::shared_mutex> lk(c.tune_mu);
     if (n == "overlap") {
         c.overlap = value != 0;
         return 0;
@@ -1808,7 +1882,7 @@ int mtgp_tuning_get(const char *name, double *value) {
     API_BEGIN
     Context &c = context();
     const std::string n = name ? name : "";
-    std::lock_guard<std::mutex> lk(c.mu);
+    std::shared_lock<std::shared_mutex> lk(c.tune_mu);
     double v;
     if (n == "overlap")
         v = c.overlap ? 1 : 0;
@@ -1975,6 +2049,7 @@ static mtgp_genset_t *genset_of(int count, const int *bits, const std::vector<co
         s->slot.push_back(slot);
         s->n.push_back(state_words(bits[g], mexp));
         s->cur.push_back(0);
+        s->epoch.push_back(0);
         s->off.push_back(bytes);
         bytes += 2 * (size_t)s->n.back() * (size_t)(bits[g] / 8);
         bytes = (bytes + 255) & ~(size_t)255;
@@ -2027,7 +2102,7 @@ int mtgp_genset_state_words(const mtgp_genset_t *s, int g) {
 }
 
 static void upload_window(mtgp_genset *s, int g, const void *w) {
-    s->cur[(size_t)g] = 0;
+    s->set_cur(g, 0);
     hip_check(hipMemcpy(s->state(g, 0), w, (size_t)s->n[(size_t)g] * (size_t)(s->bits[(size_t)g] / 8),
                         hipMemcpyHostToDevice),
               "hipMemcpy state");
@@ -2105,6 +2180,8 @@ static void device_init(mtgp_genset *s, const uint64_t *seeds, const uint64_t *k
     (void)hipFree(d_items);
     if (d_keys)
         (void)hipFree(d_keys);
+    for (int g = 0; g < G; g++)
+        s->epoch[(size_t)g]++;
 }
 
 int mtgp_genset_init_seed_device(mtgp_genset_t *s, const uint64_t *seeds, void *stream) {
@@ -2174,6 +2251,7 @@ mtgp_plan_t *mtgp_plan_create(mtgp_genset_t *s, const mtgp_request_t *reqs, int 
     std::unique_ptr<mtgp_plan> p(new mtgp_plan());
     p->set = s;
     p->plan.build(*s->ctx, s->refs(), std::vector<mtgp_request_t>(reqs, reqs + nreq), segment);
+    p->record_targets();
     return p.release();
     API_END(nullptr)
 }
@@ -2181,15 +2259,34 @@ mtgp_plan_t *mtgp_plan_create(mtgp_genset_t *s, const mtgp_request_t *reqs, int 
 int mtgp_plan_launch(mtgp_plan_t *p, void *stream) {
     API_BEGIN
     p->plan.launch((hipStream_t)stream);
+    p->launched = true;
     return 0;
     API_END(-1)
 }
 
 int mtgp_plan_commit(mtgp_plan_t *p) {
     API_BEGIN
-    for (int g : p->plan.commits)
-        p->set->cur[(size_t)g] ^= 1;
+    const auto &cm = p->plan.commits;
+    if (cm.empty())
+        return 0;  // nothing moves, or already committed
+    if (!p->launched)
+        throw Error("plan was never launched since it was built or rebased: its states are not written");
+    for (size_t i = 0; i < cm.size(); i++)
+        if (p->set->epoch[(size_t)cm[i]] != p->epochs[i])
+            throw Error("generator " + std::to_string(cm[i]) +
+                        ": state changed since the plan was built or rebased (another commit, a jump, "
+                        "set_state, init or substreams); rebase the plan and launch it again");
+    for (size_t i = 0; i < cm.size(); i++)
+        p->set->set_cur(cm[i], p->out_slot[i]);
     p->plan.commits.clear();
+    return 0;
+    API_END(-1)
+}
+
+int mtgp_plan_rebase(mtgp_plan_t *p, void *stream) {
+    API_BEGIN
+    p->plan.retarget(p->set->refs(), (hipStream_t)stream);
+    p->record_targets();
     return 0;
     API_END(-1)
 }
@@ -2201,6 +2298,8 @@ int mtgp_plan_launch_part(mtgp_plan_t *p, int part, void *stream) {
     if (part != 0 && part != 1)
         throw Error("part must be 0 (start-state jumps) or 1 (generation)");
     p->plan.launch_part(part, (hipStream_t)stream);
+    if (part == 1)
+        p->launched = true;
     return 0;
     API_END(-1)
 }
@@ -2229,6 +2328,7 @@ int mtgp_plan_capture(mtgp_plan_t *p, void *stream) {
 int mtgp_plan_launch_graph(mtgp_plan_t *p, void *stream) {
     API_BEGIN
     p->plan.launch_graph((hipStream_t)stream);
+    p->launched = true;
     return 0;
     API_END(-1)
 }
@@ -2236,6 +2336,7 @@ int mtgp_plan_launch_graph(mtgp_plan_t *p, void *stream) {
 int mtgp_plan_autotune(mtgp_plan_t *p, void *stream, int reps, int *layout, int *blocks_per_cu, float *best_ms) {
     API_BEGIN
     const float ms = p->plan.autotune((hipStream_t)stream, reps, layout, blocks_per_cu);
+    p->launched = true;  // its trials wrote the outputs and next states
     if (best_ms)
         *best_ms = ms;
     return 0;
@@ -2253,6 +2354,31 @@ int mtgp_plan_stats(const mtgp_plan_t *p, int *items, int *launches, double *pla
     return 0;
 }
 
+int mtgp_plan_launch_info(const mtgp_plan_t *p, int k, int *bits, int *cls, int *layout, int *grid, int *items,
+                          char *kernel, size_t cap) {
+    API_BEGIN
+    const auto &L = p->plan.mlaunches;
+    if (k < 0)
+        return (int)L.size();
+    if (k >= (int)L.size())
+        throw Error("launch index out of range");
+    const Plan::Launch &l = L[(size_t)k];
+    if (bits)
+        *bits = l.bits;
+    if (cls)
+        *cls = l.cls;
+    if (layout)
+        *layout = l.layout;
+    if (grid)
+        *grid = l.grid;
+    if (items)
+        *items = l.count;
+    if (kernel && put_string(kernel_name(l.bits, l.cls, l.layout), kernel, cap) < 0)
+        return -1;
+    return (int)L.size();
+    API_END(-1)
+}
+
 int mtgp_genset_generate(mtgp_genset_t *s, const mtgp_request_t *reqs, int nreq, void *stream) {
     API_BEGIN
     mtgp_plan p;
@@ -2261,7 +2387,7 @@ int mtgp_genset_generate(mtgp_genset_t *s, const mtgp_request_t *reqs, int nreq,
     p.plan.launch((hipStream_t)stream);
     hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
     for (int g : p.plan.commits)
-        s->cur[(size_t)g] ^= 1;
+        s->set_cur(g, s->cur[(size_t)g] ^ 1);
     return 0;
     API_END(-1)
 }
@@ -2273,7 +2399,7 @@ int mtgp_genset_jump(mtgp_genset_t *s, int g, const char *hex) {
     const int cur = s->cur[(size_t)g];
     device_jump(*s->ctx, s->slot[(size_t)g], s->bits[(size_t)g], s->mexp[(size_t)g], s->state(g, cur),
                 s->state(g, cur ^ 1), hex);
-    s->cur[(size_t)g] ^= 1;
+    s->set_cur(g, cur ^ 1);
     return 0;
     API_END(-1)
 }
@@ -2290,87 +2416,142 @@ int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) 
     Context &c = *s->ctx;
     auto mod = modulus_for(c, s->slot[0]);
     J = mod->reduce_any(J);
-    // c_i = J^i mod P, chained in blocks so that host threads share the work.
+    // Substream t = J^t applied to the base, built on the device by
+    // doubling: level m makes t in [2^m, 2^(m+1)) as J^(2^m) applied to
+    // substream t - 2^m (the binary decomposition the reference's
+    // substream samples use over their 4^k * 3^162 tables,
+    // openCL-sample/mtgp32-jump.cl:697-731).  The host squares J once per
+    // level; each level is one batched jump launch of one shared
+    // polynomial.  Small levels split every jump into partial jumps
+    // (chunk ranges) so the launch still spreads over the chip; a level
+    // reads its inputs as XORs of those partial windows, and a final
+    // launch of the identity polynomial writes every substream's window
+    // into the set's slot 0.
+    const int bits = s->bits[0], cls = c.class_of(s->slot[0]);
     const int pw = (s->mexp[0] + 31) / 32;
-    std::vector<std::vector<uint32_t>> polys((size_t)G);
-    const int blk = 64;
-    const int nblk = (G + blk - 1) / blk;
-    std::vector<gf2::Poly> starts((size_t)nblk);
+    const size_t wbytes = (size_t)s->n[0] * (size_t)(bits / 8);
+    const size_t stride = align256(wbytes);
+    int levels = 0;
+    while ((1 << levels) < G)
+        levels++;
+    std::vector<uint64_t> masks;
+    std::vector<int> mask_off((size_t)levels + 1), chunks((size_t)levels + 1);
     {
-        gf2::Poly Jb(mod->words(), 0);
-        Jb[0] = 1;
-        for (int k = 0; k < blk; k++)
-            Jb = mod->mulmod(Jb, J);  // J^blk
-        gf2::Poly cur(mod->words(), 0);
-        cur[0] = 1;
-        for (int b = 0; b < nblk; b++) {
-            starts[(size_t)b] = cur;
-            cur = mod->mulmod(cur, Jb);
+        gf2::Poly Jm = J;
+        for (int m = 0; m < levels; m++) {
+            if (m)
+                Jm = mod->sqrmod(Jm);
+            const std::vector<uint32_t> w = gf2::to_words32(Jm, pw);
+            mask_off[(size_t)m] = (int)masks.size();
+            chunks[(size_t)m] = append_jump_masks({&w}, masks);
+        }
+        // the identity polynomial (coefficient 0): copies the input window
+        std::vector<uint32_t> one((size_t)pw, 0);
+        one[0] = 1;
+        mask_off[(size_t)levels] = (int)masks.size();
+        chunks[(size_t)levels] = append_jump_masks({&one}, masks);
+    }
+    // Arena: substream t's window as parts[t] partial windows.
+    std::vector<int> parts((size_t)G, 1);
+    std::vector<size_t> arena_off((size_t)G, 0);
+    const int occ = c.jump_blocks_per_cu[bits == 64][cls];
+    size_t arena = 0;
+    for (int m = 0; m < levels; m++) {
+        const int first = 1 << m, n_items = std::min(first, G - first);
+        const int want = (c.cus * c.jump_waves_per_cu + n_items - 1) / n_items;
+        const int fit = std::max(1, c.cus * occ / n_items);
+        const int P = std::max(1, std::min({want, fit, chunks[(size_t)m], c.jump_max_parts}));
+        for (int t = first; t < first + n_items; t++) {
+            parts[(size_t)t] = P;
+            arena_off[(size_t)t] = arena;
+            arena += (size_t)P * stride;
         }
     }
-    parallel_for(nblk, [&](int b) {
-        gf2::Poly cur = starts[(size_t)b];
-        for (int g = b * blk; g < std::min(G, (b + 1) * blk); g++) {
-            polys[(size_t)g] = gf2::to_words32(cur, pw);
-            cur = mod->mulmod(cur, J);
+    std::vector<JumpItem> items;
+    std::vector<int> level_first((size_t)levels + 2, 0);
+    auto in_of = [&](int t, JumpItem &ji) {
+        // base (t == 0) or the partial windows of substream t, as offsets:
+        // bit 63 marks an arena offset (fixed up below)
+        if (t == 0) {
+            ji.state_in = 0;
+            ji.in_parts = 1;
+        } else {
+            ji.state_in = (1ull << 63) | arena_off[(size_t)t];
+            ji.in_parts = parts[(size_t)t];
+            ji.in_part_bytes = stride;
         }
-    });
-    // Base window to the device, then one jump item per generator.
-    const size_t wbytes = (size_t)s->n[0] * (size_t)(s->bits[0] / 8);
-    void *d_base = nullptr;
-    uint64_t *d_polys = nullptr;
-    uint32_t *d_q = nullptr;
-    JumpItem *d_items = nullptr;
-    auto cleanup = [&] {
-        (void)hipFree(d_base);
-        (void)hipFree(d_polys);
-        (void)hipFree(d_q);
-        (void)hipFree(d_items);
     };
-    try {
-        hip_check(hipMalloc(&d_base, wbytes), "hipMalloc");
-        hip_check(hipMemcpy(d_base, base, wbytes, hipMemcpyHostToDevice), "hipMemcpy");
-        // Jump items of kJumpBatch consecutive generators; their slot-0
-        // states are equally spaced in the state buffer.
-        const size_t stride = G > 1 ? (size_t)((char *)s->state(1, 0) - (char *)s->state(0, 0)) : wbytes;
-        std::vector<uint64_t> masks;
-        std::vector<JumpItem> items;
-        for (int g0 = 0; g0 < G; g0 += kJumpBatch) {
-            const int nb = std::min(kJumpBatch, G - g0);
-            std::vector<const std::vector<uint32_t> *> ps;
-            for (int b = 0; b < nb; b++)
-                ps.push_back(&polys[(size_t)(g0 + b)]);
-            JumpItem ji{};
-            ji.state_in = (uint64_t)(uintptr_t)d_base;
-            ji.masks = masks.size() * 8;
-            ji.c0 = 0;
-            ji.c1 = append_jump_masks(ps, masks);
-            ji.states_out = (uint64_t)(uintptr_t)s->state(g0, 0);
-            ji.nb = nb;
-            ji.param = s->slot[0];
-            ji.state_bytes = (int32_t)stride;
-            items.push_back(ji);
+    for (int m = 0; m < levels; m++) {
+        level_first[(size_t)m] = (int)items.size();
+        const int first = 1 << m, n_items = std::min(first, G - first);
+        for (int j = 0; j < n_items; j++) {
+            const int t = first + j;
+            for (int q = 0; q < parts[(size_t)t]; q++) {
+                JumpItem ji{};
+                in_of(j, ji);
+                ji.masks = (uint64_t)mask_off[(size_t)m] * 8;
+                ji.states_out = (1ull << 63) | (arena_off[(size_t)t] + (size_t)q * stride);
+                ji.nb = 1;
+                ji.c0 = (int32_t)((long)chunks[(size_t)m] * q / parts[(size_t)t]);
+                ji.c1 = (int32_t)((long)chunks[(size_t)m] * (q + 1) / parts[(size_t)t]);
+                ji.param = s->slot[0];
+                ji.state_bytes = (int32_t)wbytes;
+                items.push_back(ji);
+            }
         }
-        hip_check(hipMalloc((void **)&d_polys, masks.size() * 8), "hipMalloc");
-        hip_check(hipMemcpy(d_polys, masks.data(), masks.size() * 8, hipMemcpyHostToDevice), "hipMemcpy");
-        for (auto &ji : items)
-            ji.masks += (uint64_t)(uintptr_t)d_polys;
-        hip_check(hipMalloc((void **)&d_items, items.size() * sizeof(JumpItem)), "hipMalloc");
-        hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(JumpItem), hipMemcpyHostToDevice), "hipMemcpy");
-        hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
-        hip_check(hipMemset(d_q, 0, 8), "hipMemset");
-        const int bits = s->bits[0], cls = c.class_of(s->slot[0]);
-        const int n_items = (int)items.size();
-        const int grid = std::min(n_items, c.cus * c.jump_blocks_per_cu[bits == 64][cls]);
-        hip_check(launch_jumps(bits, cls, d_items, n_items, c.d_params, d_q, grid, 0, 0), "jump kernel launch");
+    }
+    level_first[(size_t)levels] = (int)items.size();
+    for (int t = 0; t < G; t++) {
+        JumpItem ji{};
+        in_of(t, ji);
+        ji.masks = (uint64_t)mask_off[(size_t)levels] * 8;
+        ji.states_out = (uint64_t)(uintptr_t)s->state(t, 0);
+        ji.nb = 1;
+        ji.c0 = 0;
+        ji.c1 = chunks[(size_t)levels];
+        ji.param = s->slot[0];
+        ji.state_bytes = (int32_t)wbytes;
+        items.push_back(ji);
+    }
+    level_first[(size_t)levels + 1] = (int)items.size();
+    // One device workspace: [base | masks | items | queue counters | arena].
+    const size_t o_masks = align256(wbytes);
+    const size_t o_items = align256(o_masks + masks.size() * 8);
+    const size_t o_queue = align256(o_items + items.size() * sizeof(JumpItem));
+    const size_t o_arena = align256(o_queue + 2 * sizeof(uint32_t) * (size_t)(levels + 1));
+    char *d = nullptr;
+    hip_check(hipMalloc((void **)&d, o_arena + arena), "hipMalloc substream workspace");
+    try {
+        const uint64_t dw = (uint64_t)(uintptr_t)d;
+        for (auto &ji : items) {
+            ji.state_in = (ji.state_in >> 63) ? dw + o_arena + (ji.state_in & ~(1ull << 63)) : dw;
+            ji.masks += dw + o_masks;
+            if (ji.states_out >> 63)
+                ji.states_out = dw + o_arena + (ji.states_out & ~(1ull << 63));
+        }
+        hip_check(hipMemcpy(d, base, wbytes, hipMemcpyHostToDevice), "hipMemcpy base");
+        hip_check(hipMemcpy(d + o_masks, masks.data(), masks.size() * 8, hipMemcpyHostToDevice), "hipMemcpy masks");
+        hip_check(hipMemcpy(d + o_items, items.data(), items.size() * sizeof(JumpItem), hipMemcpyHostToDevice),
+                  "hipMemcpy items");
+        hip_check(hipMemset(d + o_queue, 0, 2 * sizeof(uint32_t) * (size_t)(levels + 1)), "hipMemset queue");
+        const JumpItem *d_items = reinterpret_cast<const JumpItem *>(d + o_items);
+        for (int m = 0; m <= levels; m++) {
+            const int f = level_first[(size_t)m], n_items = level_first[(size_t)m + 1] - f;
+            if (!n_items)
+                continue;
+            const int grid = std::min(n_items, c.cus * occ);
+            hip_check(launch_jumps(bits, cls, d_items + f, n_items, c.d_params,
+                                   reinterpret_cast<uint32_t *>(d + o_queue) + 2 * m, grid, 0, 0),
+                      "jump kernel launch");
+        }
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (...) {
-        cleanup();
+        (void)hipFree(d);
         throw;
     }
-    cleanup();
+    (void)hipFree(d);
     for (int g = 0; g < G; g++)
-        s->cur[(size_t)g] = 0;
+        s->set_cur(g, 0);
     return 0;
     API_END(-1)
 }

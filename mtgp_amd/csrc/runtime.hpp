@@ -5,6 +5,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -112,8 +113,14 @@ struct Context {
         DevParam rec;
         unsigned char sha1[20];
     };
+    // Reserved to param_capacity - kNumSets when the context is made, so
+    // param() references stay valid while slot_of appends under `mu`.
     std::vector<Custom> customs;
     std::mutex mu;
+    // Guards the planner defaults above (layout, layout_one_round, bpc and
+    // the tuning knobs): plan builds read them under a shared lock,
+    // mtgp_tuning_set and the autotuner write them under an exclusive one.
+    mutable std::shared_mutex tune_mu;
 
     std::map<std::pair<const void *, int>, size_t> reserve_cache;
     // Dynamic LDS that limits `fn` to `per_cu` resident workgroups per CU
@@ -279,6 +286,15 @@ struct Plan {
     // class), run by launch_part (serial order).
     std::vector<Launch> mlaunches, mjlaunches;
     std::vector<int> commits;       // generators whose state moves
+    std::vector<int> all_commits;   // the same as built (commit clears `commits`)
+    // Which generator each item and jump item (in their device order) reads
+    // and writes, so retarget() can point the work list at other state
+    // slots: item_io bit 0 = state_in is the generator's window (else a
+    // jumped start state), bit 1 = state_out is the generator's next window.
+    std::vector<int32_t> item_gen, jitem_gen;
+    std::vector<uint8_t> item_io;
+    size_t o_items = 0, o_jitems = 0;  // their offsets in the workspace
+    hipEvent_t ev_upload = nullptr;    // the last upload from h_stage
     // Device copies: one grow-only workspace [items | segs | polys | masks |
     // jump items], the jumped-state arena and the work-queue counters.
     char *d_work = nullptr;
@@ -321,6 +337,14 @@ struct Plan {
         o.segment = segment;
         build(c, gens, reqs, o);
     }
+    // Points the work list at new generator windows (gens[g].state_in /
+    // state_out): the same cells, jump polynomials and outputs, continuing
+    // from other states.  Patches the items' and jump items' state addresses
+    // and uploads them on `s`; no polynomial work.
+    void retarget(const std::vector<GenRef> &gens, hipStream_t s);
+    // Waits until the last upload from the pinned staging copy has run, so
+    // the copy can be rewritten.
+    void wait_upload();
     // Sizes the grow-only buffers up front (a plan that is rebuilt often
     // then never frees, which would synchronise the device).
     void reserve(size_t work_bytes, size_t jstates_bytes);
@@ -358,10 +382,18 @@ struct Plan {
 }  // namespace mtgp
 
 // A device generator set (mtgp_genset_t): two state slots per generator;
-// `cur` names the slot holding the current window.
+// `cur` names the slot holding the current window.  Every change of a
+// generator's state (init, set_state, a jump, substreams, a plan's commit,
+// a served batch) bumps its `epoch`; a plan records the epochs it was built
+// (or rebased) against and refuses to commit over a newer state.
 struct mtgp_genset {
     mtgp::Context *ctx;
     std::vector<int> bits, mexp, slot, n, cur;
+    std::vector<uint64_t> epoch;
+    void set_cur(int g, int which) {
+        cur[(size_t)g] = which;
+        epoch[(size_t)g]++;
+    }
     // Parameter record of each generator (the one of its word size is used).
     std::vector<mtgp32_params_fast_t> p32;
     std::vector<mtgp64_params_fast_t> p64;

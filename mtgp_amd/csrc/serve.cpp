@@ -272,7 +272,7 @@ struct mtgp_server {
         B.plan.launch(streams[(size_t)s]);
         hip_check(hipEventRecord(B.done, streams[(size_t)s]), "hipEventRecord");
         for (int g : B.plan.commits)
-            set->cur[(size_t)g] ^= 1;
+            set->set_cur(g, set->cur[(size_t)g] ^ 1);
         for (int g : B.gens)
             last_batch[(size_t)g] = bi;
         B.stream = s;

@@ -53,8 +53,8 @@ def devparam(bits, rec):
 
 
 JUMP_ITEM = np.dtype([("state_in", "<u8"), ("masks", "<u8"), ("states_out", "<u8"), ("nb", "<i4"),
-                      ("c0", "<i4"), ("c1", "<i4"), ("param", "<i4"), ("state_bytes", "<i4"), ("pad0", "<i4"),
-                      ("pad", "<u8", (2,))])
+                      ("c0", "<i4"), ("c1", "<i4"), ("param", "<i4"), ("state_bytes", "<i4"), ("in_parts", "<i4"),
+                      ("in_part_bytes", "<u8"), ("pad", "<u8")])
 assert JUMP_ITEM.itemsize == 64
 BATCH, CHUNK = 4, 1024
 

@@ -143,6 +143,28 @@ def test_batched_jumps(emu, oracle, bits, mexp):
 
 
 @pytest.mark.parametrize("bits,mexp", [(32, 11213), (64, 23209)])
+def test_batched_jump_from_split_base(emu, oracle, bits, mexp):
+    """JumpItem.in_parts: the base window is the XOR of partial windows (a
+    split jump's result), as the substream doubling chains them."""
+    p = oracle.params(bits, mexp, 6)
+    w = oracle.init_state(p, 4242)
+    rng = np.random.default_rng(7)
+    wd = np.uint32 if bits == 32 else np.uint64
+    parts = rng.integers(0, np.iinfo(wd).max, (3, p.n), dtype=wd)
+    parts[2] = w ^ parts[0] ^ parts[1]
+    cs = sorted(set(rng.integers(0, mexp, size=mexp // 2).tolist()) | {mexp - 1})
+    masks, nchunks = pyemu.jump_masks([cs])
+    out = np.zeros(p.n, wd)
+    arr = np.zeros(1, pyemu.JUMP_ITEM)
+    for f, v in dict(state_in=parts.ctypes.data, masks=masks.ctypes.data, states_out=out.ctypes.data, nb=1, c0=0,
+                     c1=nchunks, param=pyemu.set_id(bits, mexp, 6), state_bytes=p.n * (bits // 8), in_parts=3,
+                     in_part_bytes=p.n * (bits // 8)).items():
+        arr[0][f] = v
+    assert emu.emu_run_jumps(bits, pyemu.kernel_class(bits, p.n), arr.ctypes.data, 1, 1) == 0
+    assert (out == oracle.jump(p, w, _hex_of(cs, cs[-1]))).all()
+
+
+@pytest.mark.parametrize("bits,mexp", [(32, 11213), (64, 23209)])
 def test_item_start_window_from_parts(emu, oracle, bits, mexp):
     """Item.parts: the start window is the XOR of partial windows."""
     p = oracle.params(bits, mexp, 4)
@@ -282,7 +304,7 @@ def test_narrow_class_items_and_jumps(emu, oracle, idx):
     jitems = np.zeros(2, pyemu.JUMP_ITEM)
     for q in range(2):
         jitems[q] = (w0.ctypes.data, masks.ctypes.data, jout[q].ctypes.data, 2, nch * q // 2, nch * (q + 1) // 2,
-                     0, 176 * 8, 0, (0, 0))
+                     0, 176 * 8, 0, 0, 0)
     assert emu.emu_run_jumps_p(64, 3, jitems.ctypes.data, 2, 2, dp.ctypes.data) == 0
     for b, q in enumerate(polys):
         want = oracle.jump(p, w0, oracle.poly_to_hex(q))
