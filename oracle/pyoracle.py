@@ -179,7 +179,7 @@ class RefLib:
         L.rs_fill.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_void_p]
         L.rs_window.argtypes = [C.c_void_p, C.c_void_p]
         L.rs_idstring.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
-        if variant in ("fast", "dc"):
+        if variant in ("fast", "dc", "dccl"):
             L.rs_set_window.argtypes = [C.c_void_p, C.c_void_p]
             L.rs_jump.argtypes = [C.c_void_p, C.c_char_p]
             L.rs_init_state.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_void_p]

@@ -23,6 +23,10 @@ Files written (all small, loaded with numpy allow_pickle=False or json):
   dc_outputs.npz        test-program sequences and stepped outputs of
                         dynamic-creator sets (reference built with
                         cuda-sample/mtgp*dc-param-11213.c)
+  dc64cl_jump.npz       the reference's 64-bit jump data (openCL-sample/:
+                        characteristic, 3^162 jump string, jump table) for
+                        set 0 of its MTGP64-DC table, that set's record and
+                        the reference mtgp64_fast_jump by the string
 
 Steps can be selected by name: make_golden.py dc_golden
 """
@@ -130,11 +134,17 @@ def reference_jump_data():
 
 
 def _batched32(g, seed, L):
+    """Two consecutive batches of generator g: L uint32 then L close_open
+    floats, twice (the second continues the stream: the next batch of a
+    rebased plan)."""
     ref = RefLib("fast", 32)
     gen = ref.new_seed(11213, g, seed)
-    a = gen.fill(0, L)
-    b = gen.fill(2, L)
-    return g, sha(a), sha(b), sha(gen.window())
+    out = []
+    for _ in range(2):
+        a = gen.fill(0, L)
+        b = gen.fill(2, L)
+        out += [sha(a), sha(b), sha(gen.window())]
+    return (g,) + tuple(out)
 
 
 def _batched64(mexp, i, key):
@@ -161,6 +171,8 @@ def streams():
     out["batched32"] = {"seeds": seeds, "lengths": lengths,
                         "uint32": [r[1] for r in res], "close_open": [r[2] for r in res],
                         "window": [r[3] for r in res]}
+    out["batched32_next"] = {"uint32": [r[4] for r in res], "close_open": [r[5] for r in res],
+                             "window": [r[6] for r in res]}
     jobs = []
     for mexp in (23209, 44497):
         for i in range(64):
@@ -197,7 +209,31 @@ def substreams():
         arrays["head_%d" % i] = out[:1024]
         arrays["sha_%d" % i] = np.array(sha(out))
     arrays["base"] = base
+    # Every substream: the first 16 bytes of the SHA-256 of its 2^20
+    # uint32 outputs and of its start window (word 0 under the mask: its
+    # other bits are outside the state and depend on how the jump
+    # polynomial is composed).
+    g = ref.new_seed(11213, 0, 777)
+    windows = []
+    for i in range(4096):
+        if i:
+            g.jump(js)
+        windows.append(g.window())
+    with ProcessPoolExecutor(8) as ex:
+        res = list(ex.map(_substream_digests, windows))
+    arrays["all_out_sha16"] = np.array([r[0] for r in res], np.uint8)
+    arrays["all_window_sha16"] = np.array([r[1] for r in res], np.uint8)
     np.savez_compressed(os.path.join(HERE, "substreams.npz"), **arrays)
+
+
+def _substream_digests(w):
+    ref = RefLib("fast", 32)
+    h = ref.new_seed(11213, 0, 777)
+    h.set_window(w)
+    out = h.fill(0, 1 << 20)
+    m = np.array(w, np.uint32)
+    m[0] &= np.uint32(int(ref.params(11213, 0)[52]))  # the record's mask
+    return (list(hashlib.sha256(out.tobytes()).digest()[:16]), list(hashlib.sha256(m.tobytes()).digest()[:16]))
 
 
 DC_SETS = {32: (0, 16, 199), 64: (0, 172, 199)}  # 16 and 172 have pos = 3
@@ -233,7 +269,34 @@ def dc_golden():
     np.savez_compressed(os.path.join(HERE, "dc_outputs.npz"), **arrays)
 
 
-STEPS = (params_digests, main_outputs, jump_steps, reference_jump_data, substreams, streams, dc_golden)
+def dc64cl_jump():
+    """The reference's 64-bit jump data (openCL-sample/): the characteristic
+    polynomial of set 0 of the OpenCL sample's MTGP64-DC-11213 table
+    (characteristic.64.11213.txt), x^(3^162) mod it (mtgp64-jump-string.h),
+    the jump table rows x^(4^k 3^162) (mtgp64-jump-table.cpp), that set's
+    record from the reference build, and the reference mtgp64_fast_jump of a
+    seeded state by the shipped string with outputs after it."""
+    lines = open(REF + "/openCL-sample/characteristic.64.11213.txt").read().split("\n")
+    assert lines[0].startswith('# "0e4c448cbacfa0f8ec70c51d24b9ffcee74843f0"')
+    char = lines[1].strip()
+    js = re.search(r'"([0-9a-f]+)"', open(REF + "/openCL-sample/mtgp64-jump-string.h").read()).group(1)
+    body = open(REF + "/openCL-sample/mtgp64-jump-table.cpp").read()
+    body = body[body.index("mtgp64_jump_table[MTGP64_JMP_TBL_SIZE * MTGP64_JTS] = {"):]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    vals = [int(v, 16) for v in re.findall(r"0x([0-9a-fA-F]{8})", body)]
+    jts = 11213 // 32 + 1
+    assert len(vals) == 6 * jts, len(vals)
+    ref = RefLib("dccl", 64)
+    rec = ref.params(11213, 0)
+    g = ref.new_seed(11213, 0, 4321)
+    g.jump(js)
+    arrays = dict(char64=np.array(char), jump3_162=np.array(js),
+                  jump_table=np.array(vals, np.uint32).reshape(6, jts), record=np.array(rec, np.uint64),
+                  window=g.window(), uint64=g.fill(0, 1000), close_open=g.fill(2, 1000))
+    np.savez_compressed(os.path.join(HERE, "dc64cl_jump.npz"), **arrays)
+
+
+STEPS = (params_digests, main_outputs, jump_steps, reference_jump_data, substreams, streams, dc_golden, dc64cl_jump)
 
 if __name__ == "__main__":
     wanted = sys.argv[1:]

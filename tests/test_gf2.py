@@ -87,3 +87,60 @@ def test_inconsistent_parameter_set_is_refused(mtgp):
     good = mtgp.params(32, 11213, 3)
     small = C.create_string_buffer(10)
     assert mtgp.lib.mtgp32_jump_hex(C.byref(good), e, 1, small, len(small)) == -1
+
+
+def params64_of_record(mtgp, rec):
+    """An mtgp64_params_fast_t from a 73-value record laid out as
+    mtgp64-fast.h:46-57 (mexp, pos, sh1, sh2, tbl, tmp_tbl, dbl_tmp_tbl,
+    mask, poly_sha1)."""
+    rec = [int(v) for v in rec]
+    p = mtgp.Params64()
+    p.mexp, p.pos, p.sh1, p.sh2 = rec[0:4]
+    for j in range(16):
+        p.tbl[j], p.tmp_tbl[j], p.dbl_tmp_tbl[j] = rec[4 + j], rec[20 + j], rec[36 + j]
+    p.mask = rec[52]
+    for j in range(20):
+        p.poly_sha1[j] = rec[53 + j]
+    return p
+
+
+def _hex_of_words(words):
+    """polytostring of a polynomial held as uint32 words (bit i of word
+    i/32, the polytoarray layout of the jump tables)."""
+    b = np.unpackbits(np.asarray(words, np.uint32).view(np.uint8), bitorder="little")
+    nib = b.reshape(-1, 4) @ np.array([1, 2, 4, 8])
+    return "".join("0123456789abcdef"[v] for v in nib).rstrip("0")
+
+
+def test_reference_64bit_jump_data(mtgp, oracle):
+    """The reference's 64-bit jump data (openCL-sample/): for set 0 of the
+    OpenCL sample's MTGP64-DC-11213 table, the library's characteristic
+    polynomial is characteristic.64.11213.txt, its x^(3^162) mod P is
+    mtgp64-jump-string.h, and its x^(4^k 3^162) mod P are the rows of
+    mtgp64-jump-table.cpp (also row k+1 = row k ^ 4 mod P in the oracle's
+    schoolbook arithmetic); the oracle's literal jump by the string equals
+    the reference mtgp64_fast_jump."""
+    d = np.load(os.path.join(GOLD, "dc64cl_jump.npz"))
+    p = params64_of_record(mtgp, d["record"])
+    assert bytes(p.poly_sha1)[:20].hex() == "0e4c448cbacfa0f8ec70c51d24b9ffcee74843f0"
+    char = mtgp.charpoly_hex(64, record=p)
+    assert char == str(d["char64"]).rstrip("0")
+    js = str(d["jump3_162"])
+    assert mtgp.jump_hex(64, record=p, steps=3**162) == js.rstrip("0")
+    words = 11213 // 64 + 2
+    P = oracle.hex_to_poly(char, words)
+    rows = d["jump_table"]
+    for k in range(6):
+        assert mtgp.jump_hex(64, record=p, steps=4**k * 3**162) == _hex_of_words(rows[k]), k
+        if k:
+            prev = oracle.hex_to_poly(_hex_of_words(rows[k - 1]), words)
+            sq = oracle.mulmod(oracle.mulmod(prev, prev, P), oracle.mulmod(prev, prev, P), P)
+            assert oracle.poly_to_hex(sq).rstrip("0") == _hex_of_words(rows[k]), k
+    from pyoracle import Oracle
+    op = Oracle.params_from_record(64, d["record"])
+    w = oracle.jump(op, oracle.init_state(op, 4321), js)
+    assert (w == d["window"]).all()
+    out, w = oracle.generate(op, w, 0, 1000)
+    assert (out == d["uint64"]).all()
+    out, w = oracle.generate(op, w, 2, 1000)
+    assert (out.view(np.uint64) == d["close_open"].view(np.uint64)).all()

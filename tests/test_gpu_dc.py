@@ -101,3 +101,19 @@ def test_dc64_genset_plan(mtgp, oracle):
         got = bufs[g].to_host(want.dtype, n)
         assert (got.view(np.uint64) == want.view(np.uint64)).all(), idx
         assert (gs.state(g) == w).all(), idx
+
+
+def test_mtgp64_fast_jump_reference_string(mtgp):
+    """The drop-in mtgp64_fast_jump on the GPU with the reference's shipped
+    64-bit jump string (openCL-sample/mtgp64-jump-string.h, x^(3^162) mod
+    the characteristic polynomial of set 0 of the OpenCL sample's
+    MTGP64-DC-11213 table, a 128-word-step set): the window and the
+    outputs after it against the reference mtgp64_fast_jump's."""
+    from test_gf2 import params64_of_record
+    d = np.load(os.path.join(GOLD, "dc64cl_jump.npz"))
+    p = params64_of_record(mtgp, d["record"])
+    g = mtgp.Generator64(record=p, seed=4321)
+    g.fast_jump(str(d["jump3_162"]))
+    assert (g.window() == d["window"]).all()
+    assert (g.fill(0, 1000) == d["uint64"]).all()
+    assert (g.fill(2, 1000).view(np.uint64) == d["close_open"].view(np.uint64)).all()

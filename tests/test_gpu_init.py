@@ -50,8 +50,7 @@ def test_device_init_array_matches_host_and_oracle(mtgp, oracle):
         assert (dev.state(g) == host.state(g)).all(), (g, specs[g], len(keys[g]))
         p = oracle.params(bits, mexp, idx)
         k = [v & 0xFFFFFFFF for v in keys[g]] if bits == 32 else keys[g]
-        if k:
-            assert (dev.state(g) == oracle.init_by_array(p, k)).all(), g
+        assert (dev.state(g) == oracle.init_by_array(p, k)).all(), (g, len(k))
 
 
 def test_device_init_dc_sets(mtgp):

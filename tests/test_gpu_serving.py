@@ -70,6 +70,66 @@ def test_server_same_alone_and_in_mix(mtgp, oracle, cell):
     assert (res[False][3] > 0).all()
 
 
+def _oracle_gen_digests(mtgp, oracle, spec, seed, reqs):
+    """Oracle digests of every request of one generator, `reqs` its
+    (trace index, kind, count, skip) in submission order; skips of 2^20 or
+    more are applied as x^skip mod P jumps."""
+    bits, mexp, idx = spec
+    p = oracle.params(bits, mexp, idx)
+    w = oracle.init_state(p, seed)
+    P = None
+    out = {}
+    for i, kind, count, skip in reqs:
+        if skip >= 1 << 20:
+            if P is None:
+                P = oracle.hex_to_poly(mtgp.charpoly_hex(bits, mexp, idx), mexp // 64 + 2)
+            w = oracle.jump(p, w, oracle.poly_to_hex(oracle.xpow_mod(skip, P)))
+        elif skip:
+            _, w = oracle.generate(p, w, 0, skip)
+        want, w = oracle.generate(p, w, kind, count)
+        out[i] = mtgp.digest64(want)
+    return out, w
+
+
+def test_server_full_serving_mix(mtgp, oracle):
+    """The benchmark's serving configuration at full size: 12,000 Poisson
+    requests at 20,000/s over 256 generators (128 MTGP32-11213, 64
+    MTGP32-23209, 64 MTGP64-23209), 2^10..2^26 numbers each, 10% with skips
+    up to 2^30, a 4 GiB output budget, through the batched server.  Every
+    request of a sample of generators -- the ones holding the largest
+    requests, skips stepped over inside a cell and skips jumped -- and their
+    final windows against the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    specs, seeds, tr = bench.serving_workload(n_requests=12000, rate=20000.0)
+    gs = _fresh_set(mtgp, specs, seeds)
+    srv = mtgp.Server(gs, arena_bytes=4 << 30)
+    print("full serving mix: replay with digests", flush=True)
+    lat, dg, wall = srv.replay(tr["t_us"], tr["gen"], tr["kind"], tr["count"], tr["skip"], digests=True)
+    srv.close()
+    gen, count, skip = tr["gen"], tr["count"].astype(np.int64), tr["skip"].astype(np.int64)
+    largest = [int(gen[i]) for i in np.argsort(-count)[:6]]
+    stepped = [int(g) for g in dict.fromkeys(gen[(skip > 0) & (skip < 1 << 17)])][:4]
+    jumped = [int(g) for g in dict.fromkeys(gen[skip >= 1 << 20])][:4]
+    sample = list(dict.fromkeys(largest + stepped + jumped))
+    assert count.max() == 1 << 26 and len(stepped) == 4 and len(jumped) == 4
+    per = {g: [] for g in sample}
+    for i in range(len(gen)):
+        if int(gen[i]) in per:
+            per[int(gen[i])].append((i, int(tr["kind"][i]), int(count[i]), int(skip[i])))
+    print("full serving mix: oracle for generators %s" % sample, flush=True)
+    with ThreadPoolExecutor(8) as ex:
+        res = dict(zip(sample, ex.map(lambda g: _oracle_gen_digests(mtgp, oracle, specs[g], seeds[g], per[g]),
+                                      sample)))
+    n_checked = 0
+    for g in sample:
+        want, w = res[g]
+        bad = [i for i in want if want[i] != int(dg[i])]
+        assert not bad, "generator %d: requests differing from the oracle: %s" % (g, bad[:10])
+        assert (gs.state(g) == w).all(), "generator %d: final window" % g
+        n_checked += len(want)
+    assert n_checked > 100
+
+
 def test_server_budget_admission(mtgp, oracle):
     """An output budget far below the offered load: requests wait for
     admission until earlier outputs are released, and every result is still

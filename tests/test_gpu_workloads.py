@@ -40,7 +40,9 @@ def streams():
 def test_batched32_workload(mtgp, streams):
     """All 128 MTGP32-11213 sets, splitmix64 seeds, L_g uint32 then L_g
     close_open floats each (the benchmark workload), in one plan: every
-    output and every final window against the reference's digests."""
+    output and every final window against the reference's digests; then
+    the plan rebased onto the committed states makes the next batch of
+    every stream, against the reference's digests of that batch."""
     gold = streams["batched32"]
     seeds, lengths = gold["seeds"], gold["lengths"]
     G = len(seeds)
@@ -67,6 +69,16 @@ def test_batched32_workload(mtgp, streams):
     assert not bad, "generators with wrong output: %s" % bad[:10]
     bad = [g for g in range(G) if _sha(gs.state(g)) != gold["window"][g]]
     assert not bad, "generators with wrong final window: %s" % bad[:10]
+    nxt = streams["batched32_next"]
+    plan.rebase()
+    plan.launch()
+    mtgp.synchronize()
+    plan.commit()
+    got = _digests(buf, parts)
+    bad = [g for g in range(G) if got[2 * g] != nxt["uint32"][g] or got[2 * g + 1] != nxt["close_open"][g]]
+    assert not bad, "second batch: generators with wrong output: %s" % bad[:10]
+    bad = [g for g in range(G) if _sha(gs.state(g)) != nxt["window"][g]]
+    assert not bad, "second batch: generators with wrong final window: %s" % bad[:10]
 
 
 @pytest.mark.parametrize("mexp", [23209, 44497])
@@ -100,11 +112,11 @@ def test_batched64_workload(mtgp, streams, mexp):
 def _substreams(mtgp, base, hexstr, count, keep=()):
     """4096 generators = substreams i of `base` by powers of `hexstr`,
     each generating `count` uint32 into one buffer.  Returns the buffer and
-    the start windows of the substreams in `keep`."""
+    the start windows of the substreams in `keep` (all with keep=None)."""
     S = 4096
     gs = mtgp.GenSet([(32, 11213, 0)] * S)
     gs.substreams(base, hexstr)
-    starts = {i: gs.state(i) for i in keep}
+    starts = {i: gs.state(i) for i in (range(S) if keep is None else keep)}
     buf = mtgp.DeviceBuffer(S * 4 * count)
     gs.generate([(i, 0, count, 0, buf.address(4 * count * i)) for i in range(S)])
     return buf, starts
@@ -113,9 +125,10 @@ def _substreams(mtgp, base, hexstr, count, keep=()):
 def test_substreams_3_162(mtgp, oracle):
     """Substream i = set 0, seed 777, jumped i times by 3^162 steps (the
     reference's jump string), for i < 4096, 2^20 uint32 each (16 GiB).
-    Sampled substreams (outputs, start windows) against the reference's own
-    repeated mtgp32_fast_jump; for a spread of i, the first 1024 outputs of
-    substream i+1 against the oracle's jump of substream i's start window."""
+    Every substream's outputs and start window against digests of the
+    reference's own repeated mtgp32_fast_jump; sampled substreams in full;
+    for a spread of i, the first 1024 outputs of substream i+1 against the
+    oracle's jump of substream i's start window."""
     sub = np.load(os.path.join(GOLD, "substreams.npz"))
     jd = np.load(os.path.join(GOLD, "reference_jump_data.npz"))
     js = str(jd["jump3_162"])
@@ -124,7 +137,21 @@ def test_substreams_3_162(mtgp, oracle):
     base = oracle.init_state(p, 777)
     rng = np.random.default_rng(5)
     chain = sorted(set(rng.integers(0, 4095, 24).tolist()) | {0, 4094})
-    buf, starts = _substreams(mtgp, base, js, count, keep=[int(i) for i in sub["indices"]] + chain)
+    buf, starts = _substreams(mtgp, base, js, count, keep=None)
+    out16 = sub["all_out_sha16"]
+    win16 = sub["all_window_sha16"]
+
+    def digest16(i):
+        out = hashlib.sha256(memoryview(buf.to_host(np.uint32, count, 4 * count * i)).cast("B")).digest()[:16]
+        w = starts[i].copy()
+        w[0] &= np.uint32(p.mask)
+        return out, hashlib.sha256(w.tobytes()).digest()[:16]
+    with ThreadPoolExecutor(8) as ex:
+        got = list(ex.map(digest16, range(4096)))
+    bad = [i for i in range(4096) if got[i][0] != bytes(out16[i])]
+    assert not bad, "substreams with wrong output: %d, first %s" % (len(bad), bad[:10])
+    bad = [i for i in range(4096) if got[i][1] != bytes(win16[i])]
+    assert not bad, "substreams with wrong start window: %d, first %s" % (len(bad), bad[:10])
     for i in sub["indices"]:
         i = int(i)
         got = buf.to_host(np.uint32, count, 4 * count * i)

@@ -205,6 +205,8 @@ def test_oracle_against_reference_build(oracle):
                 if bits == 32:
                     key = key.astype(np.uint32)
                 assert (ref.new_array(mexp, idx, key).window() == oracle.init_by_array(p, key)).all()
+                # the empty key (init_by_array with length 0)
+                assert (ref.new_array(mexp, idx, key[:0]).window() == oracle.init_by_array(p, key[:0])).all()
                 s = bytes(rng.integers(1, 256, int(rng.integers(1, 50)), dtype=np.uint8))
                 assert (ref.new_str(mexp, idx, s).window() == oracle.init_by_str(p, s)).all()
                 # the independent "ref" formulation (mtgp32-ref.c) agrees too
