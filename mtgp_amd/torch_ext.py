@@ -42,15 +42,21 @@ if len(_rt) > 1:
 
 # Output element type per (word size, kind): the bit patterns of uint32 /
 # uint64 are exposed as int32 / int64 tensors where torch lacks the unsigned
-# type.
+# type.  The floating-point kinds ([1,2) included) fill float32 / float64
+# tensors; [1,2) also fills an integer tensor of the same width with its
+# raw bits.
 _UINT32 = getattr(torch, "uint32", torch.int32)
 _UINT64 = getattr(torch, "uint64", torch.int64)
 
 
 def _dtypes(bits, kind):
-    if kind in (KIND_INT, KIND_CLOSE1_OPEN2):
-        return (_UINT32, torch.int32) if bits == 32 else (_UINT64, torch.int64)
-    return (torch.float32,) if bits == 32 else (torch.float64,)
+    ints = (_UINT32, torch.int32) if bits == 32 else (_UINT64, torch.int64)
+    floats = (torch.float32,) if bits == 32 else (torch.float64,)
+    if kind == KIND_INT:
+        return ints
+    if kind == KIND_CLOSE1_OPEN2:
+        return floats + ints
+    return floats
 
 
 class TensorGenSet(M.GenSet):

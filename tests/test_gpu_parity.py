@@ -39,6 +39,24 @@ def test_fill_sequence_all_kinds(mtgp, oracle, bits, mexp, index):
         assert_same(gen.window(), w, "window after kind %d n %d" % (kind, n))
 
 
+def test_close1_open2_is_floating_point(mtgp, oracle):
+    """[1,2) comes back as float32 / float64 (mtgp*_genrand_close1_open2
+    returns a float / double), or as raw bits into an unsigned array."""
+    for bits, mexp in ((32, 11213), (64, 23209)):
+        gen = (mtgp.Generator32 if bits == 32 else mtgp.Generator64)(mexp, 5, seed=77)
+        p = oracle.params(bits, mexp, 5)
+        w = oracle.init_state(p, 77)
+        got = gen.fill(mtgp.KIND_CLOSE1_OPEN2, 3000)
+        want, w = oracle.generate(p, w, 1, 3000)
+        assert got.dtype == (np.float32 if bits == 32 else np.float64)
+        assert_same(got, want, "close1_open2 MTGP%d" % bits)
+        assert got.min() >= 1.0 and got.max() < 2.0
+        raw = np.empty(1000, np.uint32 if bits == 32 else np.uint64)
+        gen.fill(mtgp.KIND_CLOSE1_OPEN2, 1000, out=raw)
+        want, w = oracle.generate(p, w, 1, 1000)
+        assert_same(raw, want, "close1_open2 raw bits MTGP%d" % bits)
+
+
 def test_config1_single_generator(mtgp, oracle):
     """MTGP32-11213 set 0, mtgp32_init(1234): 2^24 uint32 then 2^24
     close1_open2 continuing the stream, bit-exact, with the final state."""

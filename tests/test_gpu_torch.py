@@ -64,12 +64,25 @@ y = g.rand(2, 500, kind=mt.KIND_OPEN_OPEN)
 e, w = o.generate(p, w, 4, 1000)
 assert tuple(y.shape) == (2, 500) and (y.cpu().numpy().reshape(-1).view(np.uint32) == e.view(np.uint32)).all()
 assert (g.get_state() == w).all()
+# [1,2) into float tensors (mtgp*_genrand_close1_open2 returns float /
+# double), and as raw bits into an integer tensor of the same width
+q = g.uniform_(torch.empty(777, device="cuda"), kind=mt.KIND_CLOSE1_OPEN2)
+e, w = o.generate(p, w, 1, 777)
+assert q.dtype == torch.float32 and (q.cpu().numpy().view(np.uint32) == e.view(np.uint32)).all()
+assert float(q.min()) >= 1.0 and float(q.max()) < 2.0
+qi = g.uniform_(torch.empty(333, dtype=torch.int32, device="cuda"), kind=mt.KIND_CLOSE1_OPEN2)
+e, w = o.generate(p, w, 1, 333)
+assert (qi.cpu().numpy().view(np.uint32) == e.view(np.uint32)).all()
+assert (g.get_state() == w).all()
 g64 = mt.Generator(64, 23209, 2, array=[1, 2, 3, 4])
 p64 = o.params(64, 23209, 2)
 w64 = o.init_by_array(p64, [1, 2, 3, 4])
 z = g64.rand(3000)
 e, w64 = o.generate(p64, w64, 2, 3000)
 assert z.dtype == torch.float64 and (z.cpu().numpy().view(np.uint64) == e.view(np.uint64)).all()
+z1 = g64.uniform_(torch.empty(999, dtype=torch.float64, device="cuda"), kind=mt.KIND_CLOSE1_OPEN2)
+e, w64 = o.generate(p64, w64, 1, 999)
+assert (z1.cpu().numpy().view(np.uint64) == e.view(np.uint64)).all()
 print("TORCH-OK %.4f" % float(f.mean()))
 """
 
