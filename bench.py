@@ -176,9 +176,58 @@ def run_gpu(args):
     serial_ms = ev[0].elapsed_ms(ev[-1])
     # Bit-exact spot check of the last step's output against the oracle.
     exact = spot_check(M, buf, seeds, lengths) if not args.no_check else None
+    kernels = plan.launches()
+    sustained = None
+    if args.cycles > 0:
+        sustained = run_sustained(M, gs, plan, buf, reqs, seeds, lengths, stream, args.cycles, not args.no_check)
     return dict(G=G, cus=cus, hbm=hbm, out_bytes=out_bytes, numbers=total_numbers, per_ms=per,
                 jump_ms=jump_ms, gen_ms=gen_ms, total_ms=total_ms, serial_ms=serial_ms, stats=stats,
-                plan_s=time.time() - t0, exact=exact)
+                plan_s=time.time() - t0, exact=exact, kernels=kernels, sustained=sustained)
+
+
+def run_sustained(M, gs, plan_a, buf_a, reqs_a, seeds, lengths, stream, cycles, check):
+    """Fresh batches back to back: two plans of the workload's shape over two
+    output buffers (double buffering).  Cycle k rebases plan k % 2 onto the
+    states cycle k-1 committed, launches it and commits, all enqueued on one
+    stream while earlier cycles run (no synchronisation between cycles), so
+    every cycle writes the NEXT L_g uint32 + L_g floats of every stream.
+    The first cycle continues from the workload's initial states."""
+    out_bytes = sum(8 * L for L in lengths)
+    buf_b = M.DeviceBuffer(out_bytes)
+    delta = buf_b.address(0) - buf_a.address(0)
+    reqs_b = [(g, k, n, sk, out + delta) for g, k, n, sk, out in reqs_a]
+    t = time.perf_counter()
+    plan_b = gs.plan(reqs_b)
+    fresh_ms = (time.perf_counter() - t) * 1e3
+    plans = [plan_a, plan_b]
+    stream.synchronize()
+    ev0, ev1 = M.Event(), M.Event()
+    host = []
+    ev0.record(stream)
+    for k in range(cycles):
+        p = plans[k % 2]
+        t = time.perf_counter()
+        if k:
+            p.rebase(stream)
+        p.launch(stream)
+        p.commit()
+        host.append((time.perf_counter() - t) * 1e3)
+    ev1.record(stream)
+    stream.synchronize()
+    ms = ev0.elapsed_ms(ev1)
+    exact = None
+    if check:
+        # the last cycle's batch (the cycles-th batch of every stream) in
+        # the buffer it went to, against the oracle jumped over the earlier
+        # batches
+        exact = spot_check(M, buf_a if (cycles - 1) % 2 == 0 else buf_b, seeds, lengths, batch=cycles - 1)
+    return {"cycles": cycles, "gbs": round(out_bytes * cycles / (ms * 1e-3) / 1e9, 2),
+            "ms_per_cycle": round(ms / cycles, 4),
+            "host_ms_per_cycle": round(float(np.mean(host[1:])) if cycles > 1 else host[0], 4),
+            "fresh_plan_ms": round(fresh_ms, 1), "bit_exact": exact,
+            "how": "cycle k: plan k%2 rebased onto the states cycle k-1 committed (its work items re-pointed, "
+                   "no polynomial work), launched, committed; enqueued back to back on one stream over two "
+                   "output buffers; fresh_plan_ms is a second mtgp_plan_create of the same shape"}
 
 
 def _time_plan(M, plan, stream, reps=3):
@@ -276,21 +325,76 @@ def run_configs(M):
     w0 = base.state(0)
     gs = M.GenSet([(32, 11213, 0)] * S)
     js = M.jump_hex(32, 11213, 0, 3 ** 162)
-    M.synchronize()
-    t = time.time()
-    gs.substreams(w0, js)
-    M.synchronize()
-    setup_ms = (time.time() - t) * 1e3
     buf = M.DeviceBuffer(S * 4 * count)
+    gs.substreams(w0, js)  # warm-up (characteristic polynomial, kernels)
     plan = gs.plan([(i, M.KIND_INT, count, 0, buf.address(4 * count * i)) for i in range(S)])
-    out["4"] = line(S * 4 * count, *_time_plan(M, plan, stream), plan.stats()["items"],
-                    substream_setup_ms=round(setup_ms, 1),
-                    workload="4096 substreams i*3^162 of MTGP32-11213 set 0 (seed 777), 2^20 uint32 each; "
-                             "substream_setup_ms (host polynomials + device jumps, wall clock) is outside ms")
+    setups = []
+    for _ in range(3):
+        M.synchronize()
+        t = time.perf_counter()
+        gs.substreams(w0, js)
+        M.synchronize()
+        setups.append((time.perf_counter() - t) * 1e3)
+    setup_ms = float(np.median(setups))
+    # the plan was built on the same slots: its cells start from the windows
+    # substreams just wrote
+    jump_ms, gen_ms = _time_plan(M, plan, stream)
+    d = line(S * 4 * count, jump_ms, gen_ms, plan.stats()["items"], substream_setup_ms=round(setup_ms, 3),
+             workload="4096 substreams i*3^162 of MTGP32-11213 set 0 (seed 777), 2^20 uint32 each, end to end: "
+                      "substream start states made on the device by doubling (12 batched jump launches of "
+                      "J^(2^m) + one copy launch; host time included, wall clock) + generation")
+    d["ms"] = round(setup_ms + jump_ms + gen_ms, 4)
+    d["gbs"] = round(S * 4 * count / (d["ms"] * 1e-3) / 1e9, 2)
+    out["4"] = d
     return out
 
 
-def spot_check(M, buf, seeds, lengths, n=1 << 16):
+def jump_latency(M, reps=20):
+    """Latency of one 3^162-step jump of MTGP32-11213 set 0 (the jump the
+    reference's test-jump32.cpp times): the drop-in mtgp32_fast_jump (host
+    state in and out, GPU jump), the library's x^(3^162) mod P (the NTL
+    calc_jump replacement), and the reference library's mtgp32_fast_jump on
+    one host core."""
+    js = M.jump_hex(32, 11213, 0, 3 ** 162)
+    g = M.Generator32(11213, 0, seed=1234)
+    g.fast_jump(js)  # warm-up
+    t = time.perf_counter()
+    for _ in range(reps):
+        g.fast_jump(js)
+    drop_in = (time.perf_counter() - t) * 1e3 / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        M.jump_hex(32, 11213, 0, 3 ** 162 + 1)
+    poly = (time.perf_counter() - t) * 1e3 / reps
+    out = {"fast_jump_ms": round(drop_in, 4), "jump_poly_ms": round(poly, 4)}
+    noalloc = getattr(M._lib, "mtgp32_fast_jump_noalloc", None)
+    if noalloc is not None:
+        noalloc.restype = C.c_int
+        noalloc.argtypes = [C.POINTER(M.Fast32), C.c_char_p]
+        noalloc(C.byref(g.g), js.encode())
+        t = time.perf_counter()
+        for _ in range(reps):
+            noalloc(C.byref(g.g), js.encode())
+        out["fast_jump_noalloc_ms"] = round((time.perf_counter() - t) * 1e3 / reps, 4)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import RefLib
+    if RefLib.available("fast", 32):
+        r = RefLib("fast", 32).new_seed(11213, 0, 1234)
+        r.jump(js)
+        t = time.perf_counter()
+        for _ in range(reps):
+            r.jump(js)
+        out["reference_cpu_fast_jump_ms"] = round((time.perf_counter() - t) * 1e3 / reps, 4)
+    out["what"] = ("one jump by 3^162 steps of MTGP32-11213 set 0 (2803-character string): fast_jump = the "
+                   "drop-in on a host mtgp32_fast_t (window up, one-wave jump, window back, synchronous); "
+                   "jump_poly = x^(3^162) mod P on the host; reference = the reference library on one core")
+    return out
+
+
+def spot_check(M, buf, seeds, lengths, n=1 << 16, batch=0):
+    """Heads and tails of a few generators' outputs in `buf` against the
+    oracle; `batch` > 0: the batch-th batch of each stream (the oracle jumps
+    over the earlier ones)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Oracle
     o = Oracle()
@@ -301,6 +405,9 @@ def spot_check(M, buf, seeds, lengths, n=1 << 16):
         if g in (0, 1, len(seeds) - 1) or L == max(lengths):
             p = o.params(32, 11213, g)
             w = o.init_state(p, seeds[g])
+            if batch:
+                P = o.hex_to_poly(M.charpoly_hex(32, 11213, g), 11213 // 64 + 2)
+                w = o.jump(p, w, o.poly_to_hex(o.xpow_mod(2 * L * batch, P)))
             want, w = o.generate(p, w, 0, min(n, L))
             got = buf.to_host(np.uint32, min(n, L), off)
             ok &= bool((got == want).all())
@@ -379,6 +486,87 @@ def cpu_baseline(seconds=8.0, chunk=1 << 22):
                       "threads (single core: %.3g numbers in %.1f s)" % (cores - 1, n, dt, cores, n1, t1)}
 
 
+def _threads_for(threads, seconds, make, step):
+    """Bytes produced by `threads` host threads in `seconds`: thread t runs
+    step(state) -> bytes on its own state make(t) until the deadline."""
+    states = [make(t) for t in range(threads)]
+    done = [0] * threads
+    deadline = time.time() + seconds
+
+    def work(t):
+        while time.time() < deadline:
+            done[t] += step(states[t])
+    th = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+    t0 = time.time()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    return sum(done), time.time() - t0
+
+
+def cpu_config_baselines(seconds=3.0):
+    """The reference library (oracle/_ref, built from the reference's own
+    sources) on the host cores, in the shape of each other configuration:
+    bounded samples of `seconds` on one core and on one thread per core (at
+    most 16); GB/s of output bytes.  None when the reference build is
+    absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import RefLib
+    if not (RefLib.available("fast", 32) and RefLib.available("fast", 64)):
+        return None
+    cores = max(1, min(16, os.cpu_count() or 1))
+    out = {"cores": cores, "kind": "reference", "seconds_per_sample": seconds}
+    r32, r64 = RefLib("fast", 32), RefLib("fast", 64)
+    # 1: the one stream (a single generator: one core)
+    n = 1 << 24
+    g = r32.new_seed(11213, 0, 1234)
+    t = time.perf_counter()
+    g.fill(0, n)
+    g.fill(1, n)
+    dt = time.perf_counter() - t
+    out["1"] = {"gbs": round(8 * n / dt / 1e9, 3), "cores": 1,
+                "sample": "the whole workload: 2^24 uint32 + 2^24 close1_open2 of set 0, seed 1234, one core"}
+    chunk = 1 << 22
+
+    def alt(bits, kind2):
+        def step(st):
+            st[0].fill(0 if st[1] % 2 == 0 else kind2, chunk)
+            st[1] += 1
+            return chunk * bits // 8
+        return step
+    # 3: MTGP64 sets with init_by_array keys, uint64 / close_open double chunks
+    for mexp in (23209, 44497):
+        def make(t, mexp=mexp):
+            return [r64.new_array(mexp, t, splitmix64(1000 * mexp + t, 4)), 0]
+        b1, t1 = _threads_for(1, seconds, make, alt(64, 2))
+        b, tt = _threads_for(cores, seconds, make, alt(64, 2))
+        out["3-%d" % mexp] = {"gbs": round(b / tt / 1e9, 3), "single_core_gbs": round(b1 / t1 / 1e9, 3),
+                              "sample": "sets 0..%d, one thread each, alternating 2^22 uint64 / 2^22 close_open "
+                                        "double chunks" % (cores - 1)}
+    # 4: substreams: one 3^162 jump (mtgp32_fast_jump) then 2^20 uint32 each
+    js = None
+    try:
+        import mtgp_amd as M
+        js = M.jump_hex(32, 11213, 0, 3 ** 162)
+    except Exception:
+        pass
+    if js:
+        def make4(t):
+            return r32.new_seed(11213, 0, 777 + t)
+
+        def step4(h):
+            h.jump(js)
+            h.fill(0, 1 << 20)
+            return 4 << 20
+        b1, t1 = _threads_for(1, seconds, make4, step4)
+        b, tt = _threads_for(cores, seconds, make4, step4)
+        out["4"] = {"gbs": round(b / tt / 1e9, 3), "single_core_gbs": round(b1 / t1 / 1e9, 3),
+                    "sample": "per substream one mtgp32_fast_jump by 3^162 steps then 2^20 uint32 (the reference's "
+                              "substream recipe), one generator per thread"}
+    return out
+
+
 def measured_traffic(path=os.path.join(ROOT, "profiles", "mtgp32_items_batched11213_pmc.txt")):
     """HBM bytes per generation launch from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each scaled
@@ -407,6 +595,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--configs", action="store_true",
                     help="time the other configurations (1, 3, 4) and print their JSON instead")
+    ap.add_argument("--no-configs", action="store_true", help="leave the other configurations out of the line")
+    ap.add_argument("--cycles", type=int, default=10,
+                    help="sustained fresh batches: rebase -> launch -> commit cycles (0 = none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every generator of the workload fits one MI355X and the "
@@ -428,6 +619,10 @@ def main():
     gen_avg = sum(r["gen_ms"]) / len(r["gen_ms"])
     alg_bytes = r["out_bytes"]
     kern_gbs = alg_bytes / (gen_avg * 1e-3) / 1e9
+    # the plan's generation launches (one per word size and state class;
+    # this workload has one)
+    kernel = max(r["kernels"], key=lambda k: k["items"])["kernel"] if r["kernels"] else None
+    traffic = measured_traffic()
     line = {
         "metric": METRIC,
         "value": round(gbs, 2),
@@ -453,10 +648,15 @@ def main():
                    "serial_ms_per_step": round(r["serial_ms"] / args.steps, 4),
                    "jump_ms": round(sum(r["jump_ms"]) / len(r["jump_ms"]), 4),
                    "generate_ms": round(gen_avg, 4)},
-        "roofline": {"bound": "hbm", "kernel": "mtgp32_items_kernel<351,3,1,1,4>", "achieved": round(kern_gbs, 2),
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(kern_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(kern_gbs / HBM_PEAK_GBS, 4),
-                     "algorithmic_bytes": alg_bytes, "traffic": measured_traffic()},
+                     "algorithmic_bytes": alg_bytes, "traffic": traffic,
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this workload, calibrated "
+                                        "against a plain copy kernel: profiles/mtgp32_items_batched11213_pmc.txt")
+                     if traffic else None},
     }
+    if r["sustained"]:
+        line["sustained"] = r["sustained"]
     if not args.no_serving:
         import mtgp_amd as M
         sv = run_serving(M, args.serving_requests, args.serving_rate)
@@ -465,8 +665,14 @@ def main():
                           "x ~ N(16, 2.5) clipped to [10, 26]; 10% skip 2^U numbers first, U ~ U[10, 30]; "
                           "4 GiB output budget")
         line["serving"] = sv
+    import mtgp_amd as M
+    line["jump_latency"] = jump_latency(M)
+    if not args.no_configs:
+        line["configs"] = run_configs(M)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
+        if not args.no_configs:
+            line["cpu_baseline"]["configs"] = cpu_config_baselines()
     print(json.dumps(line), flush=True)
 
 
