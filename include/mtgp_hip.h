@@ -119,6 +119,23 @@ int mtgp32_jump_hex(const mtgp32_params_fast_t *params, const uint64_t *steps, i
 int mtgp64_jump_hex(const mtgp64_params_fast_t *params, const uint64_t *steps, int nwords,
                     char *buf, size_t cap);
 
+/* ------------------------------------------------ no-allocation jump
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same jump and the same resulting state as
+ * mtgp32_fast_jump, but the window, coefficient masks, work item and queue
+ * counters the GPU needs live in one static device array of the library
+ * (about 44 KiB per GPU, held until the process ends) instead of per-call
+ * hipMalloc; the call makes no device allocation or free.  The string may
+ * hold at most 16384 hex characters (x^s mod P of every MTGP32 table fits);
+ * a longer one, a non-hex character or a set that fits no device kernel
+ * returns -1 with mtgp_last_error() set and the state unchanged.  The
+ * parameter record is used as given (its poly_sha1 is not checked).
+ * Synchronous on the null stream of the current device.  All calls on one
+ * GPU share that array: calls that overlap on one GPU (several host threads)
+ * race and must be serialised by the caller. */
+int mtgp32_fast_jump_noalloc(mtgp32_fast_t *mtgp32, const char *jump_string);
+
 /* -------------------------------------------- device generator sets
  * A set of generators whose states stay in device memory.  Generator g
  * uses parameter set index[g] of the table for (bits[g], mexp[g]). */

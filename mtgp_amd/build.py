@@ -20,7 +20,7 @@ ROCM_INC = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "include")
 
 HOST_SOURCES = ["host.cpp", "gf2x.cpp"]
 HIP_HOST_SOURCES = ["runtime.cpp", "serve.cpp"]
-KERNEL_SOURCES = ["kernels.hip", "init_kernels.hip"]
+KERNEL_SOURCES = ["kernels.hip", "init_kernels.hip", "noalloc.hip"]
 
 # Packed FP32 instructions are kept out of all device code.
 NO_PK_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
