@@ -386,7 +386,8 @@ def jump_latency(M, reps=20):
             r.jump(js)
         out["reference_cpu_fast_jump_ms"] = round((time.perf_counter() - t) * 1e3 / reps, 4)
     out["what"] = ("one jump by 3^162 steps of MTGP32-11213 set 0 (2803-character string): fast_jump = the "
-                   "drop-in on a host mtgp32_fast_t (window up, one-wave jump, window back, synchronous); "
+                   "drop-in on a host mtgp32_fast_t (window up, the jump split over 11 waves by chunk range and "
+                   "their partial windows XORed by a second launch, window back, synchronous); "
                    "jump_poly = x^(3^162) mod P on the host; reference = the reference library on one core")
     return out
 

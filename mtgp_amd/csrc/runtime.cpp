@@ -1642,8 +1642,11 @@ struct mtgp_plan {
     }
 };
 
-// Applies a jump polynomial (hex) to device window `in`, result in `out`,
-// with the batched jump kernel (one jump).
+// Applies a jump polynomial (hex) to device window `in`, result in `out`.
+// A single jump on one wave is latency-bound (the fold of every chunk of
+// the polynomial in sequence), so it is split into partial jumps over chunk
+// ranges, one wave each, whose windows a second launch XORs together (the
+// identity polynomial with JumpItem.in_parts) into `out`.
 static void device_jump(Context &c, int slot, int bits, int mexp, const void *in, void *out, const char *hex) {
     gf2::Poly P;
     if (!gf2::from_hex(hex, P))
@@ -1652,37 +1655,58 @@ static void device_jump(Context &c, int slot, int bits, int mexp, const void *in
     // string, whatever its degree (the kernel runs the sequence that far).
     const std::vector<uint32_t> words = gf2::to_words32(P, std::max<int>(1, (int)((strlen(hex) * 4 + 31) / 32)));
     std::vector<uint64_t> masks;
-    JumpItem ji{};
-    ji.c0 = 0;
-    ji.c1 = append_jump_masks({&words}, masks);
-    ji.state_in = (uint64_t)(uintptr_t)in;
-    ji.states_out = (uint64_t)(uintptr_t)out;
-    ji.nb = 1;
-    ji.param = slot;
-    ji.state_bytes = state_words(bits, mexp) * (bits / 8);
-    uint64_t *d_masks = nullptr;
-    JumpItem *d_item = nullptr;
-    uint32_t *d_q = nullptr;
-    auto cleanup = [&] {
-        (void)hipFree(d_masks);
-        (void)hipFree(d_item);
-        (void)hipFree(d_q);
-    };
+    const int chunks = append_jump_masks({&words}, masks);
+    const size_t id_off = masks.size();
+    const std::vector<uint32_t> one(1, 1u);
+    append_jump_masks({&one}, masks);
+    const int parts = std::max(1, std::min(chunks, c.jump_max_parts));
+    const int wbytes = state_words(bits, mexp) * (bits / 8);
+    const size_t stride = (wbytes + 255) & ~(size_t)255;
+    // One workspace: [masks | items | 2 queue pairs | partial windows].
+    const size_t o_items = (masks.size() * 8 + 255) & ~(size_t)255;
+    const size_t o_queue = o_items + (((size_t)(parts + 1) * sizeof(JumpItem) + 255) & ~(size_t)255);
+    const size_t o_parts = o_queue + 256;
+    char *d = nullptr;
+    hip_check(hipMalloc((void **)&d, o_parts + (size_t)parts * stride), "hipMalloc jump workspace");
     try {
-        hip_check(hipMalloc((void **)&d_masks, masks.size() * 8), "hipMalloc");
-        hip_check(hipMemcpy(d_masks, masks.data(), masks.size() * 8, hipMemcpyHostToDevice), "hipMemcpy");
-        ji.masks = (uint64_t)(uintptr_t)d_masks;
-        hip_check(hipMalloc((void **)&d_item, sizeof(JumpItem)), "hipMalloc");
-        hip_check(hipMemcpy(d_item, &ji, sizeof(JumpItem), hipMemcpyHostToDevice), "hipMemcpy");
-        hip_check(hipMalloc((void **)&d_q, 8), "hipMalloc");
-        hip_check(hipMemset(d_q, 0, 8), "hipMemset");
-        hip_check(launch_jumps(bits, c.class_of(slot), d_item, 1, c.d_params, d_q, 1, 0, 0), "jump kernel launch");
+        std::vector<JumpItem> items((size_t)parts + 1);
+        for (int q = 0; q < parts; q++) {
+            JumpItem &ji = items[(size_t)q];
+            ji.state_in = (uint64_t)(uintptr_t)in;
+            ji.masks = (uint64_t)(uintptr_t)d;
+            ji.states_out = (uint64_t)(uintptr_t)(d + o_parts + (size_t)q * stride);
+            ji.nb = 1;
+            ji.c0 = (int32_t)((long)chunks * q / parts);
+            ji.c1 = (int32_t)((long)chunks * (q + 1) / parts);
+            ji.param = slot;
+            ji.state_bytes = wbytes;
+        }
+        JumpItem &cj = items[(size_t)parts];
+        cj.state_in = (uint64_t)(uintptr_t)(d + o_parts);
+        cj.in_parts = parts;
+        cj.in_part_bytes = stride;
+        cj.masks = (uint64_t)(uintptr_t)(d + id_off * 8);
+        cj.states_out = (uint64_t)(uintptr_t)out;
+        cj.nb = 1;
+        cj.c0 = 0;
+        cj.c1 = 1;
+        cj.param = slot;
+        cj.state_bytes = wbytes;
+        hip_check(hipMemcpy(d, masks.data(), masks.size() * 8, hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(d + o_items, items.data(), items.size() * sizeof(JumpItem), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+        hip_check(hipMemset(d + o_queue, 0, 4 * sizeof(uint32_t)), "hipMemset");
+        const int cls = c.class_of(slot);
+        const JumpItem *d_items = reinterpret_cast<const JumpItem *>(d + o_items);
+        uint32_t *d_q = reinterpret_cast<uint32_t *>(d + o_queue);
+        hip_check(launch_jumps(bits, cls, d_items, parts, c.d_params, d_q, parts, 0, 0), "jump kernel launch");
+        hip_check(launch_jumps(bits, cls, d_items + parts, 1, c.d_params, d_q + 2, 1, 0, 0), "jump kernel launch");
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (...) {
-        cleanup();
+        (void)hipFree(d);
         throw;
     }
-    cleanup();
+    (void)hipFree(d);
 }
 
 template <class Gen, class Word>
