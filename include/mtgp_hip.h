@@ -123,9 +123,10 @@ int mtgp64_jump_hex(const mtgp64_params_fast_t *params, const uint64_t *steps, i
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): the same jump and the same resulting state as
- * mtgp32_fast_jump, but the window, coefficient masks, work item and queue
- * counters the GPU needs live in one static device array of the library
- * (about 44 KiB per GPU, held until the process ends) instead of per-call
+ * mtgp32_fast_jump, but the window, coefficient masks, partial windows of
+ * the split jump, work items and queue counters the GPU needs live in one
+ * static device array of the library
+ * (about 133 KiB per GPU, held until the process ends) instead of per-call
  * hipMalloc; the call makes no device allocation or free.  The string may
  * hold at most 16384 hex characters (x^s mod P of every MTGP32 table fits);
  * a longer one, a non-hex character or a set that fits no device kernel

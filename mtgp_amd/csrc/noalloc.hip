@@ -4,8 +4,9 @@
 // serving loops, harnesses that forbid hipMalloc).
 //
 // mtgp32_fast_jump (runtime.cpp) allocates, per call, device memory for the
-// window in and out, the per-residue coefficient masks, the JumpItem and the
-// work-queue counters of the batched jump kernel.  Here those live in
+// window in and out, the per-residue coefficient masks, the partial windows
+// of the split jump, the JumpItems and the work-queue counters of the
+// batched jump kernel.  Here those live in
 // `scratch`, a file-scope __device__ array of kScratchBytes: each GPU holds
 // its own copy from module load until the process ends.  Host code never
 // passes the array's name to the runtime (on the host it names a stand-in,
@@ -30,16 +31,20 @@ constexpr int kMaxWindow = 1391;
 // Longest jump string: kMaxChunks chunks of kJumpChunk coefficients
 // (16384 hex characters, enough for x^s mod P of every MTGP32 table).
 constexpr int kMaxChunks = 64;
-constexpr size_t kQueueOff = 0;                                       // 2 x uint32_t
-constexpr size_t kItemOff = 256;                                      // JumpItem
-constexpr size_t kParamOff = 512;                                     // DevParam
-constexpr size_t kMaskOff = 768;                                      // chunks x 64 x uint64_t
-constexpr size_t kWinInOff = kMaskOff + (size_t)kMaxChunks * 64 * 8;  // window in
+// A jump is split into at most kMaxParts partial jumps over chunk ranges
+// (one wave each), whose windows a second launch XORs together.
+constexpr int kMaxParts = 16;
 constexpr size_t kWinBytes = ((size_t)kMaxWindow * 4 + 255) & ~(size_t)255;
-constexpr size_t kWinOutOff = kWinInOff + kWinBytes;                  // window out
+constexpr size_t kQueueOff = 0;                                          // 2 pairs of uint32_t
+constexpr size_t kItemOff = 256;                                         // kMaxParts + 1 JumpItems
+constexpr size_t kParamOff = kItemOff + (((kMaxParts + 1) * 64 + 255) & ~(size_t)255);  // DevParam
+constexpr size_t kIdMaskOff = kParamOff + 256;                           // identity masks (1 chunk)
+constexpr size_t kMaskOff = kIdMaskOff + 64 * 8;                         // chunks x 64 x uint64_t
+constexpr size_t kWinInOff = kMaskOff + (size_t)kMaxChunks * 64 * 8;     // window in
+constexpr size_t kPartsOff = kWinInOff + kWinBytes;                      // partial windows
+constexpr size_t kWinOutOff = kPartsOff + (size_t)kMaxParts * kWinBytes;  // window out
 constexpr size_t kScratchBytes = kWinOutOff + kWinBytes;
-static_assert(sizeof(JumpItem) <= kParamOff - kItemOff && sizeof(DevParam) <= kMaskOff - kParamOff,
-              "scratch layout");
+static_assert(sizeof(JumpItem) == 64 && sizeof(DevParam) <= kIdMaskOff - kParamOff, "scratch layout");
 
 }  // namespace
 
@@ -119,27 +124,49 @@ void jump_noalloc(mtgp32_fast_t *g, const char *hex) {
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     uint8_t *base = scratch_ptr(dev);
-    JumpItem ji{};
-    ji.state_in = (uint64_t)(uintptr_t)(base + kWinInOff);
-    ji.masks = (uint64_t)(uintptr_t)(base + kMaskOff);
-    ji.states_out = (uint64_t)(uintptr_t)(base + kWinOutOff);
-    ji.nb = 1;
-    ji.c0 = 0;
-    ji.c1 = chunks;
-    ji.param = 0;
-    ji.state_bytes = N * 4;
+    const int parts = chunks < kMaxParts ? chunks : kMaxParts;
+    JumpItem ji[kMaxParts + 1];
+    memset(ji, 0, sizeof ji);
+    for (int q = 0; q < parts; q++) {
+        ji[q].state_in = (uint64_t)(uintptr_t)(base + kWinInOff);
+        ji[q].masks = (uint64_t)(uintptr_t)(base + kMaskOff);
+        ji[q].states_out = (uint64_t)(uintptr_t)(base + kPartsOff + (size_t)q * kWinBytes);
+        ji[q].nb = 1;
+        ji[q].c0 = chunks * q / parts;
+        ji[q].c1 = chunks * (q + 1) / parts;
+        ji[q].param = 0;
+        ji[q].state_bytes = N * 4;
+    }
+    // the identity polynomial (coefficient 0) over the XOR of the parts
+    JumpItem &cj = ji[parts];
+    cj.state_in = (uint64_t)(uintptr_t)(base + kPartsOff);
+    cj.in_parts = parts;
+    cj.in_part_bytes = kWinBytes;
+    cj.masks = (uint64_t)(uintptr_t)(base + kIdMaskOff);
+    cj.states_out = (uint64_t)(uintptr_t)(base + kWinOutOff);
+    cj.nb = 1;
+    cj.c0 = 0;
+    cj.c1 = 1;
+    cj.param = 0;
+    cj.state_bytes = N * 4;
+    uint64_t id_masks[64];
+    memset(id_masks, 0, sizeof id_masks);
+    id_masks[0] = 1;
     hip_check(hipMemcpy(base + kWinInOff, w, (size_t)N * 4, hipMemcpyHostToDevice), "hipMemcpy window");
     hip_check(hipMemcpy(base + kMaskOff, masks, (size_t)chunks * 64 * sizeof(uint64_t), hipMemcpyHostToDevice),
               "hipMemcpy masks");
-    hip_check(hipMemcpy(base + kItemOff, &ji, sizeof(ji), hipMemcpyHostToDevice), "hipMemcpy item");
+    hip_check(hipMemcpy(base + kIdMaskOff, id_masks, sizeof id_masks, hipMemcpyHostToDevice), "hipMemcpy masks");
+    hip_check(hipMemcpy(base + kItemOff, ji, (size_t)(parts + 1) * sizeof(JumpItem), hipMemcpyHostToDevice),
+              "hipMemcpy items");
     hip_check(hipMemcpy(base + kParamOff, &rec, sizeof(rec), hipMemcpyHostToDevice), "hipMemcpy param");
-    // The kernel's work-queue pair must be zero on entry (it leaves it zero,
-    // but a launch that failed part-way may not have).
-    hip_check(hipMemsetAsync(base + kQueueOff, 0, 2 * sizeof(uint32_t), 0), "hipMemsetAsync queue");
-    hip_check(launch_jumps(32, cls, reinterpret_cast<const JumpItem *>(base + kItemOff), 1,
-                           reinterpret_cast<const DevParam *>(base + kParamOff),
-                           reinterpret_cast<uint32_t *>(base + kQueueOff), 1, 0, 0),
-              "jump kernel launch");
+    // The kernels' work-queue pairs must be zero on entry (a launch leaves
+    // its pair zero, but one that failed part-way may not have).
+    hip_check(hipMemsetAsync(base + kQueueOff, 0, 4 * sizeof(uint32_t), 0), "hipMemsetAsync queue");
+    const JumpItem *items = reinterpret_cast<const JumpItem *>(base + kItemOff);
+    const DevParam *param = reinterpret_cast<const DevParam *>(base + kParamOff);
+    uint32_t *queue = reinterpret_cast<uint32_t *>(base + kQueueOff);
+    hip_check(launch_jumps(32, cls, items, parts, param, queue, parts, 0, 0), "jump kernel launch");
+    hip_check(launch_jumps(32, cls, items + parts, 1, param, queue + 2, 1, 0, 0), "jump kernel launch");
     hip_check(hipMemcpy(w, base + kWinOutOff, (size_t)N * 4, hipMemcpyDeviceToHost), "hipMemcpy window back");
     memcpy(g->status->array, w, (size_t)N * 4);
     g->status->idx = N - 1;
