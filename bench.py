@@ -326,7 +326,11 @@ def run_configs(M):
     gs = M.GenSet([(32, 11213, 0)] * S)
     js = M.jump_hex(32, 11213, 0, 3 ** 162)
     buf = M.DeviceBuffer(S * 4 * count)
-    gs.substreams(w0, js)  # warm-up (characteristic polynomial, kernels)
+    M.synchronize()
+    t = time.perf_counter()
+    gs.substreams(w0, js)  # first call: level polynomials and the work list built, then run
+    M.synchronize()
+    first_ms = (time.perf_counter() - t) * 1e3
     plan = gs.plan([(i, M.KIND_INT, count, 0, buf.address(4 * count * i)) for i in range(S)])
     setups = []
     for _ in range(3):
@@ -340,9 +344,11 @@ def run_configs(M):
     # substreams just wrote
     jump_ms, gen_ms = _time_plan(M, plan, stream)
     d = line(S * 4 * count, jump_ms, gen_ms, plan.stats()["items"], substream_setup_ms=round(setup_ms, 3),
+             substream_setup_first_ms=round(first_ms, 3),
              workload="4096 substreams i*3^162 of MTGP32-11213 set 0 (seed 777), 2^20 uint32 each, end to end: "
-                      "substream start states made on the device by doubling (12 batched jump launches of "
-                      "J^(2^m) + one copy launch; host time included, wall clock) + generation")
+                      "substream start states made on the device in radix-4 levels (6 batched jump launches of "
+                      "J^(a 4^m) + one combine launch; host time included, wall clock, median of 3 calls after the "
+                      "first, which also builds the work list: substream_setup_first_ms) + generation")
     d["ms"] = round(setup_ms + jump_ms + gen_ms, 4)
     d["gbs"] = round(S * 4 * count / (d["ms"] * 1e-3) / 1e9, 2)
     out["4"] = d

@@ -1403,4 +1403,30 @@ hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, int grid, hip
     return hipGetLastError();
 }
 
+namespace {
+// One workgroup per JumpItem: the window at states_out (state_bytes) is the
+// XOR of the in_parts partial windows at state_in, in_part_bytes apart.
+__global__ __launch_bounds__(64) void combine_kernel(const JumpItem *__restrict__ items) {
+    const JumpItem &I = items[blockIdx.x];
+    const gu32 *in = reinterpret_cast<const gu32 *>(I.state_in);
+    gu32 *out = reinterpret_cast<gu32 *>(I.states_out);
+    const int words = I.state_bytes / 4;
+    const int parts = I.in_parts > 1 ? I.in_parts : 1;
+    const uint64_t stride = I.in_part_bytes / 4;
+    for (int j = threadIdx.x; j < words; j += kWave) {
+        uint32_t v = in[j];
+        for (int q = 1; q < parts; q++)
+            v ^= in[(uint64_t)q * stride + (uint64_t)j];
+        out[j] = v;
+    }
+}
+}  // namespace
+
+hipError_t launch_combine(const JumpItem *items, int n_items, hipStream_t s) {
+    if (n_items <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(combine_kernel, dim3(n_items), dim3(kWave), 0, s, items);
+    return hipGetLastError();
+}
+
 }  // namespace mtgp

@@ -54,5 +54,9 @@ hipError_t launch_init(const InitItem *items, int n_items, hipStream_t s);
 hipError_t launch_fill(void *dst, uint64_t n_words, uint32_t value, int waves, hipStream_t s);
 // Copies n16 16-byte units src -> dst (device memory, 16-byte aligned).
 hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, int grid, hipStream_t s);
+// Combines split windows: for each JumpItem (one workgroup each), the
+// window of state_bytes at states_out becomes the XOR of the in_parts
+// windows at state_in, in_part_bytes apart (JumpItem.masks unused).
+hipError_t launch_combine(const JumpItem *items, int n_items, hipStream_t s);
 
 }  // namespace mtgp

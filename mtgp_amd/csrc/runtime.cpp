@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -2434,146 +2435,157 @@ int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) 
     for (int g = 1; g < G; g++)
         if (s->slot[(size_t)g] != s->slot[0])
             throw Error("substreams need one parameter set for all generators");
-    gf2::Poly J;
-    if (!gf2::from_hex(hex, J))
-        throw Error("jump string contains a non-hex character");
+    if (!hex || !base)
+        throw Error("NULL base window or jump string");
     Context &c = *s->ctx;
-    auto mod = modulus_for(c, s->slot[0]);
-    J = mod->reduce_any(J);
-    // Substream t = J^t applied to the base, built on the device by
-    // doubling: level m makes t in [2^m, 2^(m+1)) as J^(2^m) applied to
-    // substream t - 2^m (the binary decomposition the reference's
-    // substream samples use over their 4^k * 3^162 tables,
-    // openCL-sample/mtgp32-jump.cl:697-731).  The host squares J once per
-    // level; each level is one batched jump launch of one shared
-    // polynomial.  Small levels split every jump into partial jumps
-    // (chunk ranges) so the launch still spreads over the chip; a level
-    // reads its inputs as XORs of those partial windows, and a final
-    // launch of the identity polynomial writes every substream's window
-    // into the set's slot 0.
     const int bits = s->bits[0], cls = c.class_of(s->slot[0]);
-    const int pw = (s->mexp[0] + 31) / 32;
     const size_t wbytes = (size_t)s->n[0] * (size_t)(bits / 8);
     const size_t stride = align256(wbytes);
-    int levels = 0;
-    while ((1 << levels) < G)
-        levels++;
-    std::vector<uint64_t> masks;
-    std::vector<int> mask_off((size_t)levels + 1), chunks((size_t)levels + 1);
-    {
-        gf2::Poly Jm = J;
-        for (int m = 0; m < levels; m++) {
-            if (m)
-                Jm = mod->sqrmod(Jm);
-            const std::vector<uint32_t> w = gf2::to_words32(Jm, pw);
-            mask_off[(size_t)m] = (int)masks.size();
-            chunks[(size_t)m] = append_jump_masks({&w}, masks);
-        }
-        // the identity polynomial (coefficient 0): copies the input window
-        std::vector<uint32_t> one((size_t)pw, 0);
-        one[0] = 1;
-        mask_off[(size_t)levels] = (int)masks.size();
-        chunks[(size_t)levels] = append_jump_masks({&one}, masks);
-    }
-    // Arena: substream t's window as parts[t] partial windows.
-    std::vector<int> parts((size_t)G, 1);
-    std::vector<size_t> arena_off((size_t)G, 0);
     const int occ = c.jump_blocks_per_cu[bits == 64][cls];
-    size_t arena = 0;
-    for (int m = 0; m < levels; m++) {
-        const int first = 1 << m, n_items = std::min(first, G - first);
-        const int want = (c.cus * c.jump_waves_per_cu + n_items - 1) / n_items;
-        const int fit = std::max(1, c.cus * occ / n_items);
-        const int P = std::max(1, std::min({want, fit, chunks[(size_t)m], c.jump_max_parts}));
-        for (int t = first; t < first + n_items; t++) {
-            parts[(size_t)t] = P;
-            arena_off[(size_t)t] = arena;
-            arena += (size_t)P * stride;
-        }
-    }
-    std::vector<JumpItem> items;
-    std::vector<int> level_first((size_t)levels + 2, 0);
-    auto in_of = [&](int t, JumpItem &ji) {
-        // base (t == 0) or the partial windows of substream t, as offsets:
-        // bit 63 marks an arena offset (fixed up below)
-        if (t == 0) {
-            ji.state_in = 0;
-            ji.in_parts = 1;
-        } else {
-            ji.state_in = (1ull << 63) | arena_off[(size_t)t];
-            ji.in_parts = parts[(size_t)t];
-            ji.in_part_bytes = stride;
-        }
-    };
-    for (int m = 0; m < levels; m++) {
-        level_first[(size_t)m] = (int)items.size();
-        const int first = 1 << m, n_items = std::min(first, G - first);
-        for (int j = 0; j < n_items; j++) {
-            const int t = first + j;
-            for (int q = 0; q < parts[(size_t)t]; q++) {
-                JumpItem ji{};
-                in_of(j, ji);
-                ji.masks = (uint64_t)mask_off[(size_t)m] * 8;
-                ji.states_out = (1ull << 63) | (arena_off[(size_t)t] + (size_t)q * stride);
-                ji.nb = 1;
-                ji.c0 = (int32_t)((long)chunks[(size_t)m] * q / parts[(size_t)t]);
-                ji.c1 = (int32_t)((long)chunks[(size_t)m] * (q + 1) / parts[(size_t)t]);
-                ji.param = s->slot[0];
-                ji.state_bytes = (int32_t)wbytes;
-                items.push_back(ji);
+    auto &W = s->sub;
+    if (!W.d || W.key != hex) {
+        // Substream t = J^t applied to the base, built on the device in
+        // radix-4 levels: level m makes t = a 4^m + j (a = 1..3, j < 4^m)
+        // as J^(a 4^m) applied to substream j -- the decomposition of t in
+        // powers of J the reference's substream samples use over their
+        // 4^k 3^162 tables (openCL-sample/mtgp32-jump.cl:697-731).  The host
+        // forms the three polynomials of each level by squarings and one
+        // product; each level is one batched jump launch.  Small levels
+        // split every jump into partial jumps (chunk ranges) so the launch
+        // still spreads over the chip; a level reads such a window as the
+        // XOR of its parts (JumpItem.in_parts), and a last launch XORs every
+        // substream's parts into the set's slot 0.
+        gf2::Poly J;
+        if (!gf2::from_hex(hex, J))
+            throw Error("jump string contains a non-hex character");
+        auto mod = modulus_for(c, s->slot[0]);
+        J = mod->reduce_any(J);
+        const int pw = (s->mexp[0] + 31) / 32;
+        constexpr int R = 4;
+        std::vector<long> pow_r{1};  // R^m
+        while (pow_r.back() < G)
+            pow_r.push_back(pow_r.back() * R);
+        const int levels = (int)pow_r.size() - 1;
+        std::vector<uint64_t> masks;
+        std::vector<std::array<int, R>> mask_off((size_t)levels), chunks((size_t)levels);
+        {
+            gf2::Poly Jm = J;  // J^(R^m)
+            for (int m = 0; m < levels; m++) {
+                gf2::Poly Pa = Jm;
+                for (int a = 1; a < R; a++) {
+                    if (a > 1)
+                        Pa = mod->mulmod(Pa, Jm);
+                    const std::vector<uint32_t> w = gf2::to_words32(Pa, pw);
+                    mask_off[(size_t)m][(size_t)a] = (int)masks.size();
+                    chunks[(size_t)m][(size_t)a] = append_jump_masks({&w}, masks);
+                }
+                Jm = mod->mulmod(Pa, Jm);  // J^(R^(m+1))
             }
         }
-    }
-    level_first[(size_t)levels] = (int)items.size();
-    for (int t = 0; t < G; t++) {
-        JumpItem ji{};
-        in_of(t, ji);
-        ji.masks = (uint64_t)mask_off[(size_t)levels] * 8;
-        ji.states_out = (uint64_t)(uintptr_t)s->state(t, 0);
-        ji.nb = 1;
-        ji.c0 = 0;
-        ji.c1 = chunks[(size_t)levels];
-        ji.param = s->slot[0];
-        ji.state_bytes = (int32_t)wbytes;
-        items.push_back(ji);
-    }
-    level_first[(size_t)levels + 1] = (int)items.size();
-    // One device workspace: [base | masks | items | queue counters | arena].
-    const size_t o_masks = align256(wbytes);
-    const size_t o_items = align256(o_masks + masks.size() * 8);
-    const size_t o_queue = align256(o_items + items.size() * sizeof(JumpItem));
-    const size_t o_arena = align256(o_queue + 2 * sizeof(uint32_t) * (size_t)(levels + 1));
-    char *d = nullptr;
-    hip_check(hipMalloc((void **)&d, o_arena + arena), "hipMalloc substream workspace");
-    try {
-        const uint64_t dw = (uint64_t)(uintptr_t)d;
+        // Arena: substream t as parts[t] partial windows.
+        std::vector<int> parts((size_t)G, 1);
+        std::vector<size_t> arena_off((size_t)G, 0);
+        size_t arena = 0;
+        for (int m = 0; m < levels; m++) {
+            const long lo = pow_r[(size_t)m], hi = std::min<long>(pow_r[(size_t)m + 1], G);
+            const int n_items = (int)(hi - lo);
+            const int want = (c.cus * c.jump_waves_per_cu + n_items - 1) / n_items;
+            const int fit = std::max(1, c.cus * occ / n_items);
+            const int P = std::max(1, std::min({want, fit, chunks[(size_t)m][1], c.jump_max_parts}));
+            for (long t = lo; t < hi; t++) {
+                parts[(size_t)t] = P;
+                arena_off[(size_t)t] = arena;
+                arena += (size_t)P * stride;
+            }
+        }
+        // Addresses are offsets into the workspace until it is allocated.
+        constexpr uint64_t kArena = 1ull << 62, kMasks = 1ull << 61;
+        auto in_of = [&](long t, JumpItem &ji) {
+            if (t == 0) {
+                ji.state_in = 0;  // the base window at the workspace start
+                ji.in_parts = 1;
+            } else {
+                ji.state_in = kArena | arena_off[(size_t)t];
+                ji.in_parts = parts[(size_t)t];
+                ji.in_part_bytes = stride;
+            }
+        };
+        std::vector<JumpItem> items;
+        W.first.clear();
+        for (int m = 0; m < levels; m++) {
+            W.first.push_back((int)items.size());
+            const long lo = pow_r[(size_t)m], hi = std::min<long>(pow_r[(size_t)m + 1], G);
+            for (long t = lo; t < hi; t++) {
+                const int a = (int)(t / lo);
+                const long j = t - a * lo;
+                const int ch = chunks[(size_t)m][(size_t)a];
+                for (int q = 0; q < parts[(size_t)t]; q++) {
+                    JumpItem ji{};
+                    in_of(j, ji);
+                    ji.masks = kMasks | ((uint64_t)mask_off[(size_t)m][(size_t)a] * 8);
+                    ji.states_out = kArena | (arena_off[(size_t)t] + (size_t)q * stride);
+                    ji.nb = 1;
+                    ji.c0 = (int32_t)((long)ch * q / parts[(size_t)t]);
+                    ji.c1 = (int32_t)((long)ch * (q + 1) / parts[(size_t)t]);
+                    ji.param = s->slot[0];
+                    ji.state_bytes = (int32_t)wbytes;
+                    items.push_back(ji);
+                }
+            }
+        }
+        W.first.push_back((int)items.size());
+        W.combine_first = (int)items.size();
+        for (long t = 0; t < G; t++) {
+            JumpItem ji{};
+            in_of(t, ji);
+            ji.states_out = (uint64_t)(uintptr_t)s->state((int)t, 0);
+            ji.state_bytes = (int32_t)wbytes;
+            items.push_back(ji);
+        }
+        W.combine_count = G;
+        const size_t o_masks = stride;
+        W.o_items = align256(o_masks + masks.size() * 8);
+        W.o_queue = align256(W.o_items + items.size() * sizeof(JumpItem));
+        const size_t o_arena = align256(W.o_queue + 2 * sizeof(uint32_t) * (size_t)levels);
+        const size_t need = o_arena + arena;
+        W.key.clear();
+        if (need > W.cap) {
+            if (W.d)
+                hip_check(hipFree(W.d), "hipFree");
+            W.d = nullptr;
+            W.cap = 0;
+            hip_check(hipMalloc((void **)&W.d, need), "hipMalloc substream workspace");
+            W.cap = need;
+        }
+        const uint64_t dw = (uint64_t)(uintptr_t)W.d;
         for (auto &ji : items) {
-            ji.state_in = (ji.state_in >> 63) ? dw + o_arena + (ji.state_in & ~(1ull << 63)) : dw;
-            ji.masks += dw + o_masks;
-            if (ji.states_out >> 63)
-                ji.states_out = dw + o_arena + (ji.states_out & ~(1ull << 63));
+            ji.state_in = (ji.state_in & kArena) ? dw + o_arena + (ji.state_in & ~kArena) : dw;
+            if (ji.masks & kMasks)
+                ji.masks = dw + o_masks + (ji.masks & ~kMasks);
+            if (ji.states_out & kArena)
+                ji.states_out = dw + o_arena + (ji.states_out & ~kArena);
         }
-        hip_check(hipMemcpy(d, base, wbytes, hipMemcpyHostToDevice), "hipMemcpy base");
-        hip_check(hipMemcpy(d + o_masks, masks.data(), masks.size() * 8, hipMemcpyHostToDevice), "hipMemcpy masks");
-        hip_check(hipMemcpy(d + o_items, items.data(), items.size() * sizeof(JumpItem), hipMemcpyHostToDevice),
+        if (!masks.empty())
+            hip_check(hipMemcpy(W.d + o_masks, masks.data(), masks.size() * 8, hipMemcpyHostToDevice),
+                      "hipMemcpy masks");
+        hip_check(hipMemcpy(W.d + W.o_items, items.data(), items.size() * sizeof(JumpItem), hipMemcpyHostToDevice),
                   "hipMemcpy items");
-        hip_check(hipMemset(d + o_queue, 0, 2 * sizeof(uint32_t) * (size_t)(levels + 1)), "hipMemset queue");
-        const JumpItem *d_items = reinterpret_cast<const JumpItem *>(d + o_items);
-        for (int m = 0; m <= levels; m++) {
-            const int f = level_first[(size_t)m], n_items = level_first[(size_t)m + 1] - f;
-            if (!n_items)
-                continue;
-            const int grid = std::min(n_items, c.cus * occ);
-            hip_check(launch_jumps(bits, cls, d_items + f, n_items, c.d_params,
-                                   reinterpret_cast<uint32_t *>(d + o_queue) + 2 * m, grid, 0, 0),
-                      "jump kernel launch");
-        }
-        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    } catch (...) {
-        (void)hipFree(d);
-        throw;
+        hip_check(hipMemset(W.d + W.o_queue, 0, 2 * sizeof(uint32_t) * (size_t)levels), "hipMemset queue");
+        W.key = hex;
     }
-    (void)hipFree(d);
+    hip_check(hipMemcpy(W.d, base, wbytes, hipMemcpyHostToDevice), "hipMemcpy base");
+    const JumpItem *d_items = reinterpret_cast<const JumpItem *>(W.d + W.o_items);
+    for (size_t k = 0; k + 1 < W.first.size(); k++) {
+        const int f = W.first[k], n_items = W.first[k + 1] - f;
+        if (!n_items)
+            continue;
+        hip_check(launch_jumps(bits, cls, d_items + f, n_items, c.d_params,
+                               reinterpret_cast<uint32_t *>(W.d + W.o_queue) + 2 * k,
+                               std::min(n_items, c.cus * occ), 0, 0),
+                  "jump kernel launch");
+    }
+    hip_check(launch_combine(d_items + W.combine_first, W.combine_count, 0), "combine launch");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     for (int g = 0; g < G; g++)
         s->set_cur(g, 0);
     return 0;

@@ -399,9 +399,22 @@ struct mtgp_genset {
     std::vector<mtgp64_params_fast_t> p64;
     std::vector<size_t> off;  // byte offset of slot pair in d_states
     char *d_states = nullptr;
+    // Device work of mtgp_genset_substreams, kept for the next call with the
+    // same jump polynomial: [base window | masks | jump items | queue
+    // counters | partial windows].
+    struct Substreams {
+        std::string key;  // jump string the work list was built for
+        char *d = nullptr;
+        size_t cap = 0;
+        std::vector<int> first;  // jump launches: items [first[k], first[k+1])
+        int combine_first = 0, combine_count = 0;
+        size_t o_items = 0, o_queue = 0;
+    } sub;
     ~mtgp_genset() {
         if (d_states)
             (void)hipFree(d_states);
+        if (sub.d)
+            (void)hipFree(sub.d);
     }
     void *state(int g, int which) {
         return d_states + off[(size_t)g] + (size_t)which * (size_t)n[(size_t)g] * (size_t)(bits[(size_t)g] / 8);
