@@ -122,6 +122,31 @@ def run_serving(M, n_requests, rate, **server_args):
     return out
 
 
+def run_serving_sweep(M, rates, seconds=0.5):
+    """The batched server alone on the serving mix at increasing arrival
+    rates (`seconds` of arrivals each): offered and served GB/s, p50 / p99
+    latency and host planning per batch, up to saturation."""
+    out = []
+    for rate in rates:
+        n = max(200, int(rate * seconds))
+        specs, seeds, tr = serving_workload(n_requests=n, rate=rate)
+        gs = M.GenSet(specs)
+        gs.init_seed(seeds)
+        srv = M.Server(gs)
+        lat, _, wall = srv.replay(tr["t_us"], tr["gen"], tr["kind"], tr["count"], tr["skip"])
+        st = srv.stats()
+        srv.close()
+        nbytes = float(sum(int(c) * (8 if k == 2 else 4) for c, k in zip(tr["count"], tr["kind"])))
+        out.append({"rate_per_s": rate, "requests": n,
+                    "offered_gbs": round(nbytes / (tr["t_us"][-1] * 1e-6) / 1e9, 2),
+                    "served_gbs": round(nbytes / (wall * 1e-6) / 1e9, 2),
+                    "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
+                    "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3),
+                    "batches": int(st["batches"]),
+                    "plan_us_per_batch": round(st["plan_us"] / max(1, st["batches"]), 1)})
+    return out
+
+
 def run_gpu(args):
     import mtgp_amd as M
     seeds, lengths = batched_workload()
@@ -599,6 +624,8 @@ def main():
                     help="run only the serving comparison and print its JSON (for profiling it alone)")
     ap.add_argument("--serving-rate", type=float, default=20000.0, help="serving requests per second")
     ap.add_argument("--serving-requests", type=int, default=12000)
+    ap.add_argument("--serving-sweep", type=str, default="",
+                    help="comma-separated arrival rates: print only the batched server's sweep over them")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--configs", action="store_true",
                     help="time the other configurations (1, 3, 4) and print their JSON instead")
@@ -613,6 +640,11 @@ def main():
     if args.configs:
         import mtgp_amd as M
         print(json.dumps({"configs": run_configs(M)}), flush=True)
+        return
+    if args.serving_sweep:
+        import mtgp_amd as M
+        rates = [float(x) for x in args.serving_sweep.split(",")]
+        print(json.dumps({"serving_sweep": run_serving_sweep(M, rates)}), flush=True)
         return
     if args.only_serving:
         import mtgp_amd as M
@@ -671,6 +703,10 @@ def main():
                           "64 MTGP64-23209); uint32 / float [1,2) / double [0,1) equally likely; 2^x numbers, "
                           "x ~ N(16, 2.5) clipped to [10, 26]; 10% skip 2^U numbers first, U ~ U[10, 30]; "
                           "4 GiB output budget")
+        # batched against plain at a rate the plain path sustains
+        sv["sustainable"] = run_serving(M, 3000, 5000.0)
+        # the batched server alone up to saturation
+        sv["sweep"] = run_serving_sweep(M, [10000.0, 20000.0, 40000.0, 80000.0, 160000.0, 320000.0])
         line["serving"] = sv
     import mtgp_amd as M
     line["jump_latency"] = jump_latency(M)

@@ -2,6 +2,8 @@
 any result.  Each request's output is compared between the batched server
 (requests coalesced and overlapping on the device) and the plain path (one
 launch per request, in arrival order), and against the oracle."""
+import os
+
 import numpy as np
 import pytest
 
@@ -43,18 +45,23 @@ def _oracle_digests(mtgp, oracle, specs, seeds, tr, which):
     return out, {g: states[g][1] for g in states}
 
 
-@pytest.mark.parametrize("cell", [1 << 12, 1 << 17])
-def test_server_same_alone_and_in_mix(mtgp, oracle, cell):
+@pytest.mark.parametrize("cell,planners", [(1 << 12, 0), (1 << 17, 0), (1 << 12, 3)])
+def test_server_same_alone_and_in_mix(mtgp, oracle, cell, planners):
     """A Poisson trace over mixed MTGP32-11213 / MTGP32-23209 / MTGP64-23209
     generators, request sizes 2^6..2^16, every kind, 10% skips: identical
     digests batched and plain, oracle digests for every request, and the
-    final states."""
+    final states; batches planned on the scheduler thread or on planner
+    threads (MTGP_SERVER_PLANNERS)."""
     specs, seeds, tr = bench.serving_workload(n_requests=400, rate=20000.0, n_gen=12, log2_mean=11.0,
                                               log2_sd=2.0, log2_min=6, log2_max=16, seed=3)
     res = {}
     for plain in (False, True):
         gs = _fresh_set(mtgp, specs, seeds)
-        srv = mtgp.Server(gs, arena_bytes=64 << 20, cell=cell, cached_cells=64, plain=plain)
+        os.environ["MTGP_SERVER_PLANNERS"] = str(planners)
+        try:
+            srv = mtgp.Server(gs, arena_bytes=64 << 20, cell=cell, cached_cells=64, plain=plain)
+        finally:
+            del os.environ["MTGP_SERVER_PLANNERS"]
         lat, dg, wall = srv.replay(tr["t_us"], tr["gen"], tr["kind"], tr["count"], tr["skip"], digests=True)
         st = srv.stats()
         srv.close()
