@@ -240,19 +240,36 @@ def run_sustained(M, gs, plan_a, buf_a, reqs_a, seeds, lengths, stream, cycles, 
     ev1.record(stream)
     stream.synchronize()
     ms = ev0.elapsed_ms(ev1)
+    # host work of one cycle alone (the loop above also waits, two cycles
+    # behind, for the GPU to free a plan's staging copy)
+    work = []
+    for k in range(4):
+        p = plans[(cycles + k) % 2]
+        stream.synchronize()
+        t = time.perf_counter()
+        p.rebase(stream)
+        p.launch(stream)
+        p.commit()
+        work.append((time.perf_counter() - t) * 1e3)
+    stream.synchronize()
+    cycles_run = cycles + 4
     exact = None
     if check:
         # the last cycle's batch (the cycles-th batch of every stream) in
         # the buffer it went to, against the oracle jumped over the earlier
         # batches
-        exact = spot_check(M, buf_a if (cycles - 1) % 2 == 0 else buf_b, seeds, lengths, batch=cycles - 1)
+        exact = spot_check(M, buf_a if (cycles_run - 1) % 2 == 0 else buf_b, seeds, lengths, batch=cycles_run - 1)
     return {"cycles": cycles, "gbs": round(out_bytes * cycles / (ms * 1e-3) / 1e9, 2),
             "ms_per_cycle": round(ms / cycles, 4),
             "host_ms_per_cycle": round(float(np.mean(host[1:])) if cycles > 1 else host[0], 4),
+            "host_work_ms_per_cycle": round(float(np.median(work)), 4),
             "fresh_plan_ms": round(fresh_ms, 1), "bit_exact": exact,
             "how": "cycle k: plan k%2 rebased onto the states cycle k-1 committed (its work items re-pointed, "
                    "no polynomial work), launched, committed; enqueued back to back on one stream over two "
-                   "output buffers; fresh_plan_ms is a second mtgp_plan_create of the same shape"}
+                   "output buffers; host_ms_per_cycle includes waiting for the GPU (the host runs at most two "
+                   "cycles ahead), host_work_ms_per_cycle is one cycle's host calls with the GPU idle; "
+                   "fresh_plan_ms is a second mtgp_plan_create of the same shape; bit_exact checks the last "
+                   "batch written against the oracle jumped over the earlier batches"}
 
 
 def _time_plan(M, plan, stream, reps=3):

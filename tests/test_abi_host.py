@@ -3,6 +3,7 @@ exported, the parameter-table symbols hold the reference's records, and the
 drop-in headers compile as C and reproduce the reference test programs'
 sequences through the library's host functions and the inline genrand
 family (no GPU involved: these are host functions)."""
+import ctypes
 import hashlib
 import json
 import os
@@ -234,6 +235,11 @@ def test_gpu_entry_points_fail_loudly_without_gpu(mtgp):
     with pytest.raises(mtgp.MTGPError):
         g.fast_jump(mtgp.jump_hex(32, 11213, 0, 12345))
     assert (g.window() == w0).all()
+    f = mtgp._lib.mtgp32_fast_jump_noalloc
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.POINTER(mtgp.Fast32), ctypes.c_char_p]
+    assert f(ctypes.byref(g.g), mtgp.jump_hex(32, 11213, 0, 12345).encode()) == -1
+    assert mtgp.last_error() and (g.window() == w0).all()
     with pytest.raises(mtgp.MTGPError):
         mtgp.GenSet([(32, 11213, 0)])
     with pytest.raises(mtgp.MTGPError):
