@@ -68,8 +68,10 @@ int mtgp_device_info(int *compute_units, size_t *hbm_bytes);
  * cells per CU chosen by a step-time model whose jump launch costs this
  * much plus the jumps' own time), "jump_waves_per_cu"
  * (MTGP_JUMP_WAVES), "jump_max_parts" (MTGP_JUMP_MAX_PARTS, most partial
- * jumps a jump is split into), "overlap" (MTGP_OVERLAP, default two-phase
- * launch).
+ * jumps a jump is split into), "jump_mfma_min" (MTGP_JUMP_MFMA_MIN: an
+ * MTGP32 generator's start-state jumps of one plan run on the matrix cores
+ * when there are at least this many, default 256; 0 = never), "overlap"
+ * (MTGP_OVERLAP, default two-phase launch).
  * They apply to plans created afterwards; set them while no other thread
  * is creating a plan.  -1 for an unknown name or a value out of range. */
 int mtgp_tuning_set(const char *name, double value);
@@ -257,6 +259,9 @@ int mtgp_plan_autotune(mtgp_plan_t *plan, void *stream, int reps, int *layout, i
 /* Work items, kernel launches of one mtgp_plan_launch (jump and
  * generation), and host planning time of a plan. */
 int mtgp_plan_stats(const mtgp_plan_t *plan, int *items, int *launches, double *plan_ms);
+/* Start-state jumps of one mtgp_plan_launch: all of them, those run on the
+ * matrix cores, and the generator groups those form. */
+int mtgp_plan_jump_stats(const mtgp_plan_t *plan, int *jumps, int *matrix_jumps, int *matrix_groups);
 /* Generation launch k of mtgp_plan_launch_part(plan, 1, ...), in order:
  * word size, state-size class, step layout, workgroups, work items and the
  * kernel's template name as profilers print it (written to kernel[cap] when

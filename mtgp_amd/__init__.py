@@ -148,6 +148,7 @@ _sig("mtgp_tuning_set", i32, C.c_char_p, C.c_double)
 _sig("mtgp_tuning_get", i32, C.c_char_p, C.POINTER(C.c_double))
 _sig("mtgp_plan_launch_part", i32, vp, i32, vp)
 _sig("mtgp_plan_stats", i32, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double))
+_sig("mtgp_plan_jump_stats", i32, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int))
 _sig("mtgp_plan_capture", i32, vp, vp)
 _sig("mtgp_plan_launch_graph", i32, vp, vp)
 _sig("mtgp_plan_autotune", i32, vp, vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float))
@@ -601,7 +602,10 @@ class Plan:
     def stats(self):
         it, ln, ms = C.c_int(), C.c_int(), C.c_double()
         _lib.mtgp_plan_stats(self.h, C.byref(it), C.byref(ln), C.byref(ms))
-        return {"items": it.value, "launches": ln.value, "plan_ms": ms.value}
+        j, mj, mg = C.c_int(), C.c_int(), C.c_int()
+        _lib.mtgp_plan_jump_stats(self.h, C.byref(j), C.byref(mj), C.byref(mg))
+        return {"items": it.value, "launches": ln.value, "plan_ms": ms.value, "jumps": j.value,
+                "matrix_jumps": mj.value, "matrix_groups": mg.value}
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -73,6 +73,47 @@ struct alignas(16) JumpItem {
 };
 static_assert(sizeof(JumpItem) == 64, "JumpItem layout");
 
+// Jumps on the matrix cores: nj jumps of one MTGP32 base window, as one
+// GF(2) product of their coefficient bits with the bit planes of the base's
+// sequence (kernels.hip, mtgp32_jump_mfma_kernel).
+//  - planes : 32 * pw words the planes kernel fills (positions [0, 32 pw)).
+//  - coef   : coefficient words, coef[w * njp + b] = word w of jump b
+//             (bit t = coefficient of x^(32 w + t)), kpad / 32 words per
+//             jump, jumps zero-padded to njp (a multiple of 32).
+//  - out    : jump b's window (n words) at out + b * out_stride bytes.
+struct alignas(16) MJumpGroup {
+    uint64_t state_in;       // base window; with in_parts > 1 the XOR of
+    uint64_t in_part_bytes;  // in_parts windows this many bytes apart
+    uint64_t planes;
+    uint64_t coef;
+    uint64_t out;
+    uint64_t out_stride;
+    int32_t nj, njp, kpad, pw;
+    int32_t param, in_parts, n, pad;
+};
+static_assert(sizeof(MJumpGroup) == 80, "MJumpGroup layout");
+
+// One wave of the matrix-core jump: jumps [r0, r0 + 32 row_tiles) of a
+// group (as far as it has them) and window words [j0, j0 + mj_words).
+struct alignas(16) MJumpItem {
+    int32_t group, r0, j0, pad;
+};
+static_assert(sizeof(MJumpItem) == 16, "MJumpItem layout");
+constexpr int kMjRowTiles = 2;  // row tiles of 32 jumps per wave, at most
+constexpr int kMjWords = 8;     // window words per wave with one row tile
+// Words per item of a launch with `row_tiles` row tiles (the same
+// accumulator count).
+inline int mj_words(int row_tiles) { return kMjWords / row_tiles; }
+// Coefficients per jump, rounded up to the 64 of one product step.
+inline int mj_kpad(int mexp) { return (mexp + 63) / 64 * 64; }
+// Plane words per bit (32 positions each) a group needs: every window the
+// products read ends below position 32 pw; even, so the planes kernel's
+// 64-position rows tile it.
+inline int mj_pw(int kpad, int n) {
+    const int pw = kpad / 32 + (n + kMjWords + 31) / 32 + 3;
+    return (pw + 1) & ~1;
+}
+
 // Device initialisation of one generator's window (mtgp*_init_state or
 // mtgp*_init_by_array computed on the GPU).
 struct alignas(16) InitItem {

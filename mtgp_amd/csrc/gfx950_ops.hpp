@@ -25,4 +25,31 @@ __device__ __forceinline__ void store_u32(store_window_t w, uint32_t off, uint32
     __builtin_amdgcn_raw_buffer_store_b32(v, w, (int)off, 0, 0);
 }
 
+// Matrix-core operands of the fp4 (e2m1) 32 x 32 x 64 product: 32 values
+// per lane in dwords 0..3 (value 8v + q in nibble q of dword v; dwords 4..7
+// unused), fp32 accumulators.
+typedef int mfma_v8i __attribute__((ext_vector_type(8)));
+typedef float mfma_v16f __attribute__((ext_vector_type(16)));
+
+// acc + A B with A 32 x 64 and B 64 x 32 in fp4, unit scales: lane L holds
+// row L % 32 of A and column L % 32 of B, both for the 32 values of K group
+// L / 32 (the same value index means the same k in both); acc[r] of lane L
+// is row 8 (r / 4) + 4 (L / 32) + r % 4, column L % 32.
+__device__ __forceinline__ mfma_v16f mfma_fp4_32x32x64(mfma_v8i a, mfma_v8i b, mfma_v16f acc) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, acc, 4, 4, 0, 127, 0, 127);
+}
+
+// v of lane (lane ^ D) within each group of 32 lanes (ds_swizzle, bit mode).
+template <int D>
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (D << 10) | 0x1f);
+}
+
+// Pins a matrix-core accumulator to its AGPRs at this point, so that the
+// copies a read-out makes come after it (one accumulator at a time).
+__device__ __forceinline__ void keep_in_agprs(mfma_v16f &acc) { asm volatile("" : "+a"(acc)); }
+
+// Bit L = p of lane L.
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
+
 }  // namespace mtgp

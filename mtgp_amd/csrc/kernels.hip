@@ -1429,4 +1429,248 @@ hipError_t launch_combine(const JumpItem *items, int n_items, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ------------------------------------------------- jumps on the matrix cores
+// The nj jumps of one base window are one GF(2) product: bit p of jump b's
+// new word j is the parity of sum_k c_b[k] * bit_p(w[k + j]) over the
+// base's sequence w (the fold of mtgp32_jump_kernel, one bit at a time).
+// With the coefficient bits and the sequence bits as fp4 values 0 and 1.0,
+// one 32 x 32 x 64 matrix-core product adds 65536 such terms exactly (the
+// counts stay below 2^24 in fp32), and the parity of each accumulator is
+// the jumped bit:
+//  * rows: 32 jumps (coefficients k in groups of 64);
+//  * columns: the 32 bits of one window word j, bit p read from the bit
+//    plane of the sequence, so a column's 64 values are the plane's bits
+//    k + j for the product step's 64 k.
+// The planes kernel writes each group's planes first: the sequence in 32 x
+// 32 bit blocks transposed in place (planes[32 x + p] holds bit p of
+// w[32 x + t] in bit t), so a plane's window of 32 + kMjWords bits at any
+// offset is a funnel shift of three words.
+namespace {
+
+constexpr uint32_t kFp4Ones = 0x22222222u;  // fp4 1.0 (0b0010) in every nibble
+
+// 32 coefficient bits (bit t = k offset t) as an fp4 operand: value 8v + q
+// is 1.0 when bit 4q + v is set.  B operands use the same order.
+__device__ __forceinline__ mfma_v8i fp4_of_bits(uint32_t c) {
+    mfma_v8i a = {0, 0, 0, 0, 0, 0, 0, 0};
+    a[0] = (int)((c << 1) & kFp4Ones);
+    a[1] = (int)(c & kFp4Ones);
+    a[2] = (int)((c >> 1) & kFp4Ones);
+    a[3] = (int)((c >> 2) & kFp4Ones);
+    return a;
+}
+
+// Low word of the 64-bit (hi:lo) shifted right by s < 32 (v_alignbit_b32).
+__device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s) {
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
+}
+
+// One round of the 32 x 32 bit transpose: lanes l and l ^ D swap the
+// off-diagonal D x D blocks of their 2D x 2D block (lo masks the low D bits
+// of every 2D).
+template <int D>
+__device__ __forceinline__ uint32_t transpose_round(uint32_t v, uint32_t l, uint32_t lo) {
+    const uint32_t p = lane_xor32<D>(v);
+    return (l & D) ? ((v & ~lo) | ((p >> D) & lo)) : ((v & lo) | ((p << D) & ~lo));
+}
+
+// Each group of 32 lanes holds a 32 x 32 bit matrix, row t in lane t;
+// returns column l of it to lane l (bit t = row t's bit l).
+__device__ __forceinline__ uint32_t transpose32(uint32_t v) {
+    const uint32_t l = threadIdx.x;
+    v = transpose_round<16>(v, l, 0x0000ffffu);
+    v = transpose_round<8>(v, l, 0x00ff00ffu);
+    v = transpose_round<4>(v, l, 0x0f0f0f0fu);
+    v = transpose_round<2>(v, l, 0x33333333u);
+    v = transpose_round<1>(v, l, 0x55555555u);
+    return v;
+}
+
+// One workgroup of kPlaneWaves waves per MJumpGroup: the base window (n
+// words), then the recursion 256 words at a time (one word per thread) in a
+// linear LDS buffer, relocated as in the generation kernels; every complete
+// row of 64 positions is transposed (one row per wave) and stored, so
+// nothing is dropped by a relocation (n >= 64).
+constexpr int kPlaneWaves = 4;
+__global__ __launch_bounds__(64 * kPlaneWaves) void mtgp32_planes_kernel(const MJumpGroup *__restrict__ groups,
+                                                                        int n_groups,
+                                                                        const DevParam *__restrict__ params) {
+    constexpr int W = kPlaneWaves;
+    constexpr int NMAX = 1391;
+    constexpr int K = 4;
+    constexpr int NJ = (NMAX + kWave * W - 1) / (kWave * W);
+    constexpr int NBUF = K * kStep + kWave * W * NJ + kWave;
+    __shared__ __attribute__((aligned(16))) uint32_t buf[NBUF];
+    __shared__ uint32_t tab[kTabWords];
+    const int t = threadIdx.x, l = t & (kWave - 1), wv = t / kWave;
+    for (int gi = blockIdx.x; gi < n_groups; gi += gridDim.x) {
+        const MJumpGroup &G = groups[gi];
+        const DevParam &D = params[G.param];
+        P32 p;
+        p.n = D.n;
+        p.pos = D.pos;
+        p.pm1 = D.pos - 1;
+        p.sh1 = D.sh1;
+        p.sh2 = D.sh2;
+        p.mask = D.mask_lo;
+        fill_tables(tab, D);
+        const gu32 *in = reinterpret_cast<const gu32 *>(G.state_in);
+        const int in_parts = G.in_parts;
+        const uint64_t in_stride = G.in_part_bytes / 4;
+        for (int j = t; j < p.n; j += kWave * W) {
+            uint32_t v = in[j];
+            for (int q = 1; q < in_parts; q++)
+                v ^= in[(uint64_t)q * in_stride + (uint64_t)j];
+            buf[j] = v;
+        }
+        wave_sync();
+        gu32 *T = reinterpret_cast<gu32 *>(G.planes);
+        const int total = 32 * G.pw;
+        int off = 0, len = p.n, done = 0;  // buf[i] = w[off + i] for i < len
+        for (;;) {
+            // complete rows, one per wave
+            const int avail = off + len < total ? off + len : total;
+            while (done + kWave <= avail) {
+                const int row = done + kWave * wv;
+                // (every wave transposes; a wave without a row of its own
+                // repeats the first one and does not store it)
+                const uint32_t v = transpose32(buf[(row + kWave <= avail ? row : done) - off + l]);
+                if (row + kWave <= avail)
+                    T[row + l] = v;
+                done += kWave * ((avail - done) / kWave < W ? (avail - done) / kWave : W);
+            }
+            if (done >= total)
+                break;
+            wave_sync();
+            if (len + kStep > NBUF) {
+                relocate32<NJ, W>(buf, len - p.n, p.n);
+                off += len - p.n;
+                len = p.n;
+            }
+            step32s<0, false, false, W, 1>(buf, tab, p, len - p.n, nullptr, 0);
+            wave_sync();
+            len += kStep;
+        }
+        wave_sync();
+    }
+}
+
+// One wave per MJumpItem: accumulators for RT x 32 jumps (the item's jumps
+// fill RT row tiles) and JW words, product steps of 64 coefficients; then
+// each accumulator's parity, gathered by ballots into window words.
+template <int RT, int JW>
+__global__ __launch_bounds__(64) void mtgp32_jump_mfma_kernel(const MJumpGroup *__restrict__ groups,
+                                                              const MJumpItem *__restrict__ items) {
+    const MJumpItem I = items[blockIdx.x];
+    const MJumpGroup &G = groups[I.group];
+    const int L = threadIdx.x, col = L & 31, kg = L >> 5;
+    const int nj = G.nj, njp = G.njp, r0 = I.r0, j0 = I.j0;
+    const gu32 *T = reinterpret_cast<const gu32 *>(G.planes);
+    const gu32 *C = reinterpret_cast<const gu32 *>(G.coef);
+    const uint32_t sh = (uint32_t)j0 & 31u;
+    mfma_v16f acc[RT][JW];
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int c = 0; c < JW; c++)
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                acc[r][c][i] = 0.0f;
+    const int nks = G.kpad >> 6;
+    // Operands of step ks: coefficient words cw[r] and plane words d0..d2;
+    // the next step's are loaded before this step's products.
+    const gu32 *cp = C + (uint64_t)kg * (uint64_t)njp + (uint64_t)(r0 + col);
+    const gu32 *tp = T + (uint64_t)(kg + (j0 >> 5)) * 32 + (uint64_t)col;
+    const uint64_t cstep = 2 * (uint64_t)njp;
+    uint32_t cw[RT], d0, d1, d2;
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+        cw[r] = cp[32 * r];
+    d0 = tp[0];
+    d1 = tp[32];
+    d2 = tp[64];
+    for (int ks = 0; ks < nks; ks++) {
+        mfma_v8i A[RT];
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+            A[r] = fp4_of_bits(cw[r]);
+        // This lane's 32 positions start at k = 64 ks + 32 kg; the column of
+        // word j0 + jj reads plane `col` from bit k + j0 + jj on.
+        const uint32_t wlo = funnel(d1, d0, sh), whi = funnel(d2, d1, sh);
+        if (ks + 1 < nks) {
+            cp += cstep;
+            tp += 64;
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+                cw[r] = cp[32 * r];
+            d0 = tp[0];
+            d1 = tp[32];
+            d2 = tp[64];
+        }
+        // Value 8v + q of word jj's operand is bit 4q + v + jj of the window
+        // (wlo, whi): nibble q of U[u] = (window >> u) masked to bit 1, with
+        // u = v + jj - 1 (U[-1] = (window << 1) masked); U[4a + r] is P_r =
+        // U[r] over the whole 64-bit window, shifted right by 4a.
+        uint32_t plo[4], phi[4], U[JW + 4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            plo[r] = funnel(whi, wlo, (uint32_t)r) & kFp4Ones;
+            phi[r] = (whi >> r) & kFp4Ones;
+        }
+        U[0] = (wlo << 1) & kFp4Ones;
+#pragma unroll
+        for (int u = 0; u < JW + 3; u++)
+            U[u + 1] = (u >> 2) == 0 ? plo[u & 3] : funnel(phi[u & 3], plo[u & 3], 4u * (uint32_t)(u >> 2));
+#pragma unroll
+        for (int jj = 0; jj < JW; jj++) {
+            const mfma_v8i B = {(int)U[jj], (int)U[jj + 1], (int)U[jj + 2], (int)U[jj + 3], 0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+                acc[r][jj] = mfma_fp4_32x32x64(A[r], B, acc[r][jj]);
+        }
+    }
+    // acc[r][jj][i] of lane L: jump r0 + 32 r + 8 (i / 4) + 4 (L / 32) + i % 4,
+    // bit L % 32 of word j0 + jj.  A ballot of the parities gives that word
+    // for two jumps (lanes 0..31 and 32..63); lanes 0 and 1 store them.
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+#pragma unroll
+        for (int jj = 0; jj < JW; jj++) {
+            // read out one accumulator at a time
+            keep_in_agprs(acc[r][jj]);
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint64_t m = wave_ballot(((uint32_t)acc[r][jj][i] & 1u) != 0);
+                const int jump = r0 + 32 * r + 8 * (i / 4) + 4 * L + i % 4, word = j0 + jj;
+                if (L < 2 && jump < nj && word < G.n) {
+                    gu32 *o = reinterpret_cast<gu32 *>(G.out + (uint64_t)jump * G.out_stride);
+                    o[word] = (uint32_t)(m >> (32 * L));
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_planes32(const MJumpGroup *groups, int n_groups, const DevParam *params, hipStream_t s) {
+    if (n_groups <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(mtgp32_planes_kernel, dim3(n_groups), dim3(kWave * kPlaneWaves), 0, s, groups, n_groups,
+                       params);
+    return hipGetLastError();
+}
+
+hipError_t launch_jump_mfma32(const MJumpGroup *groups, const MJumpItem *items, int n_items, int row_tiles,
+                              hipStream_t s) {
+    if (n_items <= 0)
+        return hipSuccess;
+    static_assert(kMjRowTiles == 2, "one kernel per row-tile count");
+    if (row_tiles < 1 || row_tiles > kMjRowTiles)
+        return hipErrorLaunchFailure;
+    auto *fn = row_tiles == 1 ? mtgp32_jump_mfma_kernel<1, kMjWords> : mtgp32_jump_mfma_kernel<2, kMjWords / 2>;
+    hipLaunchKernelGGL(fn, dim3(n_items), dim3(kWave), 0, s, groups, items);
+    return hipGetLastError();
+}
+
 }  // namespace mtgp

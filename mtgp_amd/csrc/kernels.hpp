@@ -46,6 +46,14 @@ const void *jump_kernel_fn(int bits, int cls);
 hipError_t launch_jumps(int bits, int cls, const JumpItem *items, int n_items, const DevParam *params,
                         uint32_t *queue, int grid, size_t dyn_lds, hipStream_t s);
 
+// Matrix-core jumps (MJumpGroup, common.hpp), MTGP32 sets with n <= 1391
+// and n - pos >= 256: first the bit planes of every group (one workgroup per
+// group), then the products (one workgroup per MJumpItem; every item of a
+// launch has jumps in exactly row_tiles of its 32-jump row tiles, 1..2).
+hipError_t launch_planes32(const MJumpGroup *groups, int n_groups, const DevParam *params, hipStream_t s);
+hipError_t launch_jump_mfma32(const MJumpGroup *groups, const MJumpItem *items, int n_items, int row_tiles,
+                              hipStream_t s);
+
 // init_kernels.hip: one wave per InitItem (n <= 1727 words).
 hipError_t launch_init(const InitItem *items, int n_items, hipStream_t s);
 

@@ -134,6 +134,11 @@ Context &context() {
         if (v > 0 && v <= 64)
             c->jump_max_parts = v;
     }
+    if (const char *e = getenv("MTGP_JUMP_MFMA_MIN")) {
+        const int v = atoi(e);
+        if (v >= 0)
+            c->jump_mfma_min = v;
+    }
     if (const char *e = getenv("MTGP_JUMP_WAVES")) {
         const int v = atoi(e);
         if (v > 0 && v <= 64)
@@ -1077,10 +1082,27 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         int nchunks;
     };
     std::vector<Group> groups;
+    // Matrix-core groups: all of a generator's jumps (both phases: the
+    // two-phase launch runs them with phase 0), one bit-plane pass each.
+    struct MGroup {
+        int g, phase;
+        std::vector<size_t> cells;
+    };
+    std::vector<MGroup> mg;
     std::vector<uint64_t> mask_host;
     std::vector<uint32_t> poly_host;
     if (masks) {
         for (int g = 0; g < G; g++) {
+            if (gens[(size_t)g].bits == 32 && c.jump_mfma_min > 0) {
+                std::vector<size_t> need;
+                for (size_t k = 0; k < cells[(size_t)g].size(); k++)
+                    if (cells[(size_t)g][k].start > 0)
+                        need.push_back(k);
+                if ((int)need.size() >= c.jump_mfma_min) {
+                    mg.push_back(MGroup{g, 0, need});
+                    continue;
+                }
+            }
             for (int ph = 0; ph < 2; ph++) {
                 std::vector<size_t> need;
                 for (size_t k = 0; k < cells[(size_t)g].size(); k++)
@@ -1148,6 +1170,79 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             jgen_list.push_back(gr.g);
         }
         arena_bytes += align256((size_t)P * nb * sbytes);
+    }
+    // Matrix-core groups: planes and jumped windows in the arena, the
+    // coefficient words of every jump transposed ([word][jump]).
+    mgroups.clear();
+    mitems.clear();
+    mgroup_gen.clear();
+    for (auto &r : mrange)
+        r = MRange{};
+    mjumps = 0;
+    std::vector<uint32_t> coef_host;
+    {
+        std::stable_sort(mg.begin(), mg.end(), [](const MGroup &a, const MGroup &b) { return a.phase < b.phase; });
+        std::vector<MJumpItem> mit[kMjRowTiles + 1][2];
+        for (const MGroup &m : mg) {
+            const GenRef &ref = gens[(size_t)m.g];
+            const int nj = (int)m.cells.size();
+            const int njp = (nj + 31) / 32 * 32;
+            const int kpad = mj_kpad(ref.mexp);
+            const int wc = kpad / 32, pw = mj_pw(kpad, ref.n);
+            const size_t sbytes = (size_t)ref.n * 4;
+            MJumpGroup gr{};
+            gr.state_in = (uint64_t)(uintptr_t)ref.state_in;
+            gr.in_parts = 1;
+            gr.planes = arena_bytes;  // fixed up below
+            arena_bytes += align256((size_t)pw * 32 * 4);
+            gr.out = arena_bytes;  // fixed up below
+            gr.out_stride = sbytes;
+            for (int b = 0; b < nj; b++)
+                slot_of[{m.g, m.cells[(size_t)b]}] = std::make_tuple((uint64_t)(arena_bytes + (size_t)b * sbytes), 1,
+                                                                     (uint64_t)0);
+            arena_bytes += align256((size_t)nj * sbytes);
+            gr.coef = (uint64_t)coef_host.size() * 4;  // fixed up below
+            coef_host.resize(coef_host.size() + (size_t)wc * (size_t)njp, 0u);
+            uint32_t *cf = coef_host.data() + gr.coef / 4;
+            for (int b = 0; b < nj; b++) {
+                const size_t k = m.cells[(size_t)b];
+                const std::vector<uint32_t> &pw32 = polys[(size_t)m.g][k];
+                const std::vector<uint32_t> w =
+                    pw32.empty() ? words_from_mask16(m16[(size_t)m.g][k], (ref.mexp + 31) / 32) : pw32;
+                for (size_t x = 0; x < w.size() && x < (size_t)wc; x++)
+                    cf[x * (size_t)njp + (size_t)b] = w[x];
+            }
+            gr.nj = nj;
+            gr.njp = njp;
+            gr.kpad = kpad;
+            gr.pw = pw;
+            gr.param = ref.slot;
+            gr.n = ref.n;
+            const int gi = (int)mgroups.size();
+            mgroups.push_back(gr);
+            mgroup_gen.push_back(m.g);
+            for (int r0 = 0; r0 < nj; r0 += 32 * kMjRowTiles) {
+                const int rt = std::min(kMjRowTiles, (nj - r0 + 31) / 32);
+                for (int j0 = 0; j0 < ref.n; j0 += mj_words(rt))
+                    mit[rt][m.phase].push_back(MJumpItem{gi, r0, j0, 0});
+            }
+            mjumps += nj;
+        }
+        int n0 = 0;
+        for (const MGroup &m : mg)
+            n0 += m.phase == 0;
+        mrange[0].g0 = 0, mrange[0].g1 = n0;
+        mrange[1].g0 = n0, mrange[1].g1 = (int)mg.size();
+        mrange[2].g0 = 0, mrange[2].g1 = (int)mg.size();
+        for (int rt = 1; rt <= kMjRowTiles; rt++) {
+            mrange[2].i0[rt] = (int)mitems.size();
+            for (int ph = 0; ph < 2; ph++) {
+                mrange[ph].i0[rt] = (int)mitems.size();
+                mitems.insert(mitems.end(), mit[rt][ph].begin(), mit[rt][ph].end());
+                mrange[ph].i1[rt] = (int)mitems.size();
+            }
+            mrange[2].i1[rt] = (int)mitems.size();
+        }
     }
     // Items, one per cell.
     std::vector<Item> list;
@@ -1249,9 +1344,13 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     const size_t o_polys = align256(o_segs + segs.size() * sizeof(Segment));
     const size_t o_masks = align256(o_polys + poly_host.size() * 4);
     const size_t o_jitems = align256(o_masks + mask_host.size() * 8);
-    const size_t work_bytes = align256(o_jitems + jitems.size() * sizeof(JumpItem));
+    const size_t o_mgroups = align256(o_jitems + jitems.size() * sizeof(JumpItem));
+    const size_t o_mitems = align256(o_mgroups + mgroups.size() * sizeof(MJumpGroup));
+    const size_t o_coef = align256(o_mitems + mitems.size() * sizeof(MJumpItem));
+    const size_t work_bytes = align256(o_coef + coef_host.size() * 4);
     this->o_items = o_items;
     this->o_jitems = o_jitems;
+    this->o_mgroups = o_mgroups;
     wait_upload();
     grow_device(d_work, work_cap, work_bytes, "hipMalloc plan workspace");
     grow_pinned(h_stage, stage_cap, work_bytes);
@@ -1264,6 +1363,8 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     const uint64_t dw = (uint64_t)(uintptr_t)d_work;
     d_items = reinterpret_cast<Item *>(d_work + o_items);
     d_jitems = reinterpret_cast<JumpItem *>(d_work + o_jitems);
+    d_mgroups = reinterpret_cast<MJumpGroup *>(d_work + o_mgroups);
+    d_mitems = reinterpret_cast<MJumpItem *>(d_work + o_mitems);
     items.clear();
     item_gen.clear();
     item_io.clear();
@@ -1283,6 +1384,11 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         ji.masks = dw + o_masks + ji.masks;
         ji.states_out = (uint64_t)(uintptr_t)d_jstates + ji.states_out;
     }
+    for (auto &gr : mgroups) {
+        gr.planes += (uint64_t)(uintptr_t)d_jstates;
+        gr.out += (uint64_t)(uintptr_t)d_jstates;
+        gr.coef += dw + o_coef;
+    }
     if (!items.empty())
         memcpy(h_stage + o_items, items.data(), items.size() * sizeof(Item));
     if (!segs.empty())
@@ -1293,6 +1399,11 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         memcpy(h_stage + o_masks, mask_host.data(), mask_host.size() * 8);
     if (!jitems.empty())
         memcpy(h_stage + o_jitems, jitems.data(), jitems.size() * sizeof(JumpItem));
+    if (!mgroups.empty()) {
+        memcpy(h_stage + o_mgroups, mgroups.data(), mgroups.size() * sizeof(MJumpGroup));
+        memcpy(h_stage + o_mitems, mitems.data(), mitems.size() * sizeof(MJumpItem));
+        memcpy(h_stage + o_coef, coef_host.data(), coef_host.size() * 4);
+    }
     hip_check(hipMemcpyAsync(d_work, h_stage, work_bytes, hipMemcpyHostToDevice, opt.stream), "plan upload");
     if (!ev_upload)
         hip_check(hipEventCreateWithFlags(&ev_upload, hipEventDisableTiming), "hipEventCreateWithFlags");
@@ -1380,7 +1491,14 @@ void Plan::retarget(const std::vector<GenRef> &gens, hipStream_t s) {
     }
     for (size_t i = 0; i < jitems.size(); i++)
         jitems[i].state_in = (uint64_t)(uintptr_t)gens.at((size_t)jitem_gen[i]).state_in;
+    for (size_t i = 0; i < mgroups.size(); i++)
+        mgroups[i].state_in = (uint64_t)(uintptr_t)gens.at((size_t)mgroup_gen[i]).state_in;
     wait_upload();
+    const size_t mb = mgroups.size() * sizeof(MJumpGroup);
+    if (mb) {
+        memcpy(h_stage + o_mgroups, mgroups.data(), mb);
+        hip_check(hipMemcpyAsync(d_work + o_mgroups, h_stage + o_mgroups, mb, hipMemcpyHostToDevice, s), "plan upload");
+    }
     const size_t ib = items.size() * sizeof(Item), jb = jitems.size() * sizeof(JumpItem);
     if (ib) {
         memcpy(h_stage + o_items, items.data(), ib);
@@ -1442,6 +1560,17 @@ void Plan::run_launches(int part, int phase, hipStream_t s) {
     const auto &I = merged ? mlaunches : launches;
     const size_t iq = merged ? 2 * (launches.size() + jlaunches.size()) : 0;
     const size_t jq = iq + 2 * I.size();
+    if (part == 0 && !mgroups.empty()) {
+        // matrix-core jumps: the planes of the phase's groups, then the
+        // products by row-tile count
+        const MRange &R = mrange[merged ? 2 : phase];
+        if (R.g1 > R.g0) {
+            hip_check(launch_planes32(d_mgroups + R.g0, R.g1 - R.g0, ctx->d_params, s), "planes kernel launch");
+            for (int rt = 1; rt <= kMjRowTiles; rt++)
+                hip_check(launch_jump_mfma32(d_mgroups, d_mitems + R.i0[rt], R.i1[rt] - R.i0[rt], rt, s),
+                          "matrix-core jump launch");
+        }
+    }
     for (size_t k = 0; part == 0 && k < J.size(); k++) {
         const Launch &L = J[k];
         if (!merged && L.phase != phase)
@@ -1604,6 +1733,8 @@ void Plan::free_device() {
     work_cap = jstates_cap = stage_cap = 0;
     d_items = nullptr;
     d_jitems = nullptr;
+    d_mgroups = nullptr;
+    d_mitems = nullptr;
 }
 
 Plan::~Plan() { free_device(); }
@@ -1891,6 +2022,12 @@ int mtgp_tuning_set(const char *name, double value) {
         c.jump_max_parts = (int)value;
         return 0;
     }
+    if (n == "jump_mfma_min") {
+        if (!(value >= 0) || value > (1 << 30))
+            throw Error("jump_mfma_min outside [0, 2^30]");
+        c.jump_mfma_min = (int)value;
+        return 0;
+    }
     double *k = tuning_knob(c, n);
     if (!k)
         throw Error("unknown tuning knob '" + n + "'");
@@ -1915,6 +2052,8 @@ int mtgp_tuning_get(const char *name, double *value) {
         v = c.jump_waves_per_cu;
     else if (n == "jump_max_parts")
         v = c.jump_max_parts;
+    else if (n == "jump_mfma_min")
+        v = c.jump_mfma_min;
     else if (double *k = tuning_knob(c, n))
         v = *k;
     else
@@ -2371,11 +2510,31 @@ int mtgp_plan_autotune(mtgp_plan_t *p, void *stream, int reps, int *layout, int 
 int mtgp_plan_stats(const mtgp_plan_t *p, int *items, int *launches, double *plan_ms) {
     if (items)
         *items = (int)p->plan.items.size();
-    if (launches)  // kernel launches of one mtgp_plan_launch
-        *launches = p->plan.two_phase() ? (int)(p->plan.launches.size() + p->plan.jlaunches.size())
-                                      : (int)(p->plan.mlaunches.size() + p->plan.mjlaunches.size());
+    if (launches) {  // kernel launches of one mtgp_plan_launch
+        const Plan &P = p->plan;
+        int m = 0;
+        for (int ph = 0; ph < 3; ph++) {
+            if ((ph == 2) == P.two_phase() || P.mrange[ph].g1 <= P.mrange[ph].g0)
+                continue;
+            m++;
+            for (int rt = 1; rt <= kMjRowTiles; rt++)
+                m += P.mrange[ph].i1[rt] > P.mrange[ph].i0[rt];
+        }
+        *launches = m + (P.two_phase() ? (int)(P.launches.size() + P.jlaunches.size())
+                                       : (int)(P.mlaunches.size() + P.mjlaunches.size()));
+    }
     if (plan_ms)
         *plan_ms = p->plan.plan_ms;
+    return 0;
+}
+
+int mtgp_plan_jump_stats(const mtgp_plan_t *p, int *jumps, int *matrix_jumps, int *matrix_groups) {
+    if (jumps)
+        *jumps = p->plan.jumps;
+    if (matrix_jumps)
+        *matrix_jumps = p->plan.mjumps;
+    if (matrix_groups)
+        *matrix_groups = (int)p->plan.mgroups.size();
     return 0;
 }
 

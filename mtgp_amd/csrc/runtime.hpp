@@ -109,6 +109,15 @@ struct Context {
     // 16 measured best: one MTGP32-11213 stream (1017 jumps) 0.195 ms per
     // jump launch against 0.25 ms at 8; batches unchanged.
     int jump_max_parts = 16;
+    // A generator's jumps of one plan run on the matrix cores (MJumpGroup)
+    // when there are at least this many and its set is an MTGP32 one;
+    // 0 = never.  Measured on MI355X: one MTGP32-11213 stream of 2^24 + 2^24
+    // numbers (1016 jumps) 0.115 ms per jump launch against 0.19 ms; the
+    // 128-generator batch (45 jumps each) 0.53 against 0.51 ms, and the
+    // generation launch after a matrix-core jump launch runs 0.2 ms slower
+    // (the fp4 products draw the power the store-bound generation runs
+    // short of), so batches keep the batched jump kernel.
+    int jump_mfma_min = 256;
     struct Custom {
         DevParam rec;
         unsigned char sha1[20];
@@ -294,6 +303,21 @@ struct Plan {
     std::vector<int32_t> item_gen, jitem_gen;
     std::vector<uint8_t> item_io;
     size_t o_items = 0, o_jitems = 0;  // their offsets in the workspace
+    // Matrix-core jumps: groups ordered by phase, items by (row tiles,
+    // phase); per phase (0, 1, 2 = merged) the group range and the item
+    // range of each row-tile count.  (A group holds all of a generator's
+    // jumps and counts as phase 0.)
+    std::vector<MJumpGroup> mgroups;
+    std::vector<MJumpItem> mitems;
+    std::vector<int32_t> mgroup_gen;
+    struct MRange {
+        int g0 = 0, g1 = 0;
+        int i0[kMjRowTiles + 1] = {}, i1[kMjRowTiles + 1] = {};
+    } mrange[3];
+    size_t o_mgroups = 0;
+    MJumpGroup *d_mgroups = nullptr;
+    MJumpItem *d_mitems = nullptr;
+    int mjumps = 0;  // jumps run on the matrix cores
     hipEvent_t ev_upload = nullptr;    // the last upload from h_stage
     // Device copies: one grow-only workspace [items | segs | polys | masks |
     // jump items], the jumped-state arena and the work-queue counters.

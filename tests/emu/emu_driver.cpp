@@ -48,3 +48,18 @@ extern "C" int emu_run_jumps_p(int bits, int cls, const mtgp::JumpItem *items, i
         return -1;
     return (queue[0] == 0 && queue[1] == 0) ? 0 : -2;
 }
+
+// Matrix-core jumps: the planes kernel over every group, then the products
+// over the items (all with row_tiles row tiles).
+extern "C" int emu_run_mjumps(const mtgp::MJumpGroup *groups, int n_groups, const mtgp::MJumpItem *items, int n_items,
+                              int row_tiles) {
+    static std::vector<mtgp::DevParam> params;
+    if (params.empty())
+        for (int k = 0; k < mtgp::kNumSets; k++)
+            params.push_back(mtgp::host_param(k));
+    if (mtgp::launch_planes32(groups, n_groups, params.data(), nullptr) != hipSuccess)
+        return -1;
+    if (mtgp::launch_jump_mfma32(groups, items, n_items, row_tiles, nullptr) != hipSuccess)
+        return -2;
+    return 0;
+}

@@ -28,6 +28,7 @@ def load():
     lib.emu_run_init.argtypes = [C.c_void_p, C.c_int]
     lib.emu_run_items_p.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     lib.emu_run_jumps_p.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    lib.emu_run_mjumps.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int]
     return lib
 
 
@@ -92,3 +93,24 @@ def run(lib, bits, items, layout, grid=None):
     cls = max(kernel_class(bits, it["n"]) for it in items) if items else 0
     rc = lib.emu_run_items(bits, cls, layout, arr.ctypes.data, len(items), grid or len(items))
     assert rc == 0, rc
+
+
+MJUMP_GROUP = np.dtype([("state_in", "<u8"), ("in_part_bytes", "<u8"), ("planes", "<u8"), ("coef", "<u8"),
+                        ("out", "<u8"), ("out_stride", "<u8"), ("nj", "<i4"), ("njp", "<i4"), ("kpad", "<i4"),
+                        ("pw", "<i4"), ("param", "<i4"), ("in_parts", "<i4"), ("n", "<i4"), ("pad", "<i4")])
+assert MJUMP_GROUP.itemsize == 80
+MJUMP_ITEM = np.dtype([("group", "<i4"), ("r0", "<i4"), ("j0", "<i4"), ("pad", "<i4")])
+MJ_WORDS = 8
+
+
+def mj_words(rt):
+    return MJ_WORDS // rt
+
+
+def mj_kpad(mexp):
+    return (mexp + 63) // 64 * 64
+
+
+def mj_pw(kpad, n):
+    pw = kpad // 32 + (n + MJ_WORDS + 31) // 32 + 3
+    return (pw + 1) & ~1

@@ -371,3 +371,75 @@ def test_multi_wave_cells(emu, oracle, bits, mexp, layout):
         assert (_bits(got) == _bits(e)).all()
         assert (raw[start + n * eb:] == 0).all() and (raw[:start] == 0).all()
     assert (cout == w).all()
+
+
+def _coef_words(cs, wc):
+    w = np.zeros(wc, np.uint32)
+    for k in cs:
+        w[k // 32] |= np.uint32(1 << (k % 32))
+    return w
+
+
+def _mjump(emu, p, mexp, idx, base, jobs, blocks, words, in_parts=1, part_bytes=0):
+    """Runs the matrix-core jump of the coefficient lists `jobs` from `base`
+    (in_parts windows) for the row blocks `blocks` ((r0, row tiles)) and the
+    window words starting at each of `words`; returns the jumped windows
+    (words not covered stay zero)."""
+    nj = len(jobs)
+    kpad = pyemu.mj_kpad(mexp)
+    wc, njp = kpad // 32, (nj + 31) // 32 * 32
+    coef = np.zeros((wc, njp), np.uint32)
+    for b, cs in enumerate(jobs):
+        coef[:, b] = _coef_words(cs, wc)
+    pw = pyemu.mj_pw(kpad, p.n)
+    planes = np.zeros(32 * pw, np.uint32)
+    out = np.zeros((nj, p.n), np.uint32)
+    g = np.zeros(1, pyemu.MJUMP_GROUP)
+    for f, v in dict(state_in=base.ctypes.data, in_part_bytes=part_bytes, planes=planes.ctypes.data,
+                     coef=coef.ctypes.data, out=out.ctypes.data, out_stride=p.n * 4, nj=nj, njp=njp, kpad=kpad,
+                     pw=pw, param=pyemu.set_id(32, mexp, idx), in_parts=in_parts, n=p.n).items():
+        g[0][f] = v
+    for r0, rt in blocks:
+        items = np.zeros(len(words), pyemu.MJUMP_ITEM)
+        items["r0"] = r0
+        items["j0"] = words
+        assert emu.emu_run_mjumps(g.ctypes.data, 1, items.ctypes.data, len(items), rt) == 0
+    return out, planes
+
+
+def _covered(words, rt, n):
+    return [(j0, min(j0 + pyemu.mj_words(rt), n)) for j0 in words]
+
+
+@pytest.mark.parametrize("mexp,idx,nj,blocks,words,parts", [
+    (11213, 5, 40, [(0, 2)], [0, 28, 29, 348], 1),
+    (23209, 1, 9, [(0, 1)], [0, 722], 2),
+    (11213, 90, 70, [(0, 2), (64, 1)], [0, 101], 1),
+])
+def test_matrix_core_jumps(emu, oracle, mexp, idx, nj, blocks, words, parts):
+    """The matrix-core jump (bit planes, fp4 products, parities): every
+    jump of a group at the window words its items cover, against the
+    oracle's literal jump -- identity and one-step polynomials, dense
+    random ones, row blocks of 1 and 2 tiles, windows starting inside a
+    plane word, the window's last words, and a base given as partial
+    windows."""
+    p = oracle.params(32, mexp, idx)
+    base = oracle.init_state(p, 77 + idx)
+    rng = np.random.default_rng(mexp + nj)
+    jobs = [[0], [1], [mexp - 1]]
+    while len(jobs) < nj:
+        jobs.append(sorted(set(rng.integers(0, mexp, size=mexp // 2).tolist())))
+    jobs = jobs[:nj]
+    src = base
+    if parts > 1:
+        src = rng.integers(0, 2**32, (parts, p.n), dtype=np.uint32)
+        src[-1] = base ^ np.bitwise_xor.reduce(src[:-1], axis=0)
+    out, _ = _mjump(emu, p, mexp, idx, src, jobs, blocks, words, in_parts=parts, part_bytes=p.n * 4)
+    for b, cs in enumerate(jobs):
+        want = oracle.jump(p, base, _hex_of(cs, max(cs)))
+        rt = next(t for r0, t in blocks if r0 <= b < r0 + 32 * t)
+        covered = np.zeros(p.n, bool)
+        for j0, j1 in _covered(words, rt, p.n):
+            assert (out[b, j0:j1] == want[j0:j1]).all(), "jump %d words %d..%d" % (b, j0, j1)
+            covered[j0:j1] = True
+        assert not out[b, ~covered].any(), "jump %d: words outside its items written" % b
