@@ -12,8 +12,10 @@
 //      the arena (first fit, 256-byte aligned, so every output starts on a
 //      fresh cache line),
 //   3. forms batches of everything admitted, one per state-size class, in
-//      arrival order -- fixing each batch's state slots, stream and
-//      dependencies -- and hands them to the planner threads, which build
+//      arrival order (the classes taking turns for the free batch slots,
+//      the one with the oldest waiting request first) -- fixing each
+//      batch's state slots, stream and dependencies -- and hands them to
+//      the planner threads, which build
 //      them side by side (Plan::build, with its upload on the batch's
 //      stream),
 //   4. launches planned batches in formation order: a batch waits
@@ -435,18 +437,28 @@ struct mtgp_server {
                     admitted.erase(admitted.begin());
                 } else {
                     // Batches of one state-size class each, in arrival
-                    // order, capped in requests and numbers, while slots last.
+                    // order, capped in requests and numbers, while slots
+                    // last: one batch per class per pass, the class with
+                    // the oldest waiting request first, so a busy class
+                    // does not take every slot from the others.
                     std::map<int, std::vector<int64_t>> by_group;
                     {
                         std::lock_guard<std::mutex> lk(mu);
                         for (int64_t tk : admitted)
                             by_group[gen_group[(size_t)reqs.at(tk).gen]].push_back(tk);
                     }
-                    std::vector<int64_t> left;
-                    for (auto &kv : by_group) {
-                        const auto &v = kv.second;
-                        size_t i = 0;
-                        while (i < v.size() && can_form()) {
+                    std::vector<std::pair<int64_t, std::vector<int64_t> *>> order;
+                    for (auto &kv : by_group)
+                        order.emplace_back(kv.second.front(), &kv.second);
+                    std::sort(order.begin(), order.end());
+                    std::vector<size_t> next(order.size(), 0);
+                    for (bool more = true; more && can_form();) {
+                        more = false;
+                        for (size_t o = 0; o < order.size() && can_form(); o++) {
+                            const auto &v = *order[o].second;
+                            size_t &i = next[o];
+                            if (i >= v.size())
+                                continue;
                             std::vector<int64_t> part;
                             uint64_t numbers = 0;
                             {
@@ -459,9 +471,12 @@ struct mtgp_server {
                                 }
                             }
                             form(part);
+                            more |= i < v.size();
                         }
-                        left.insert(left.end(), v.begin() + (long)i, v.end());
                     }
+                    std::vector<int64_t> left;
+                    for (size_t o = 0; o < order.size(); o++)
+                        left.insert(left.end(), order[o].second->begin() + (long)next[o], order[o].second->end());
                     std::sort(left.begin(), left.end());
                     admitted.swap(left);
                 }
