@@ -645,13 +645,13 @@ mtgp_server_t *mtgp_server_create(mtgp_genset_t *set, const mtgp_server_config_t
         s->streams.push_back(st);
         s->stream_load.push_back(0);
     }
-    // Planner threads (MTGP_SERVER_PLANNERS; default 0 = plan on the
-    // scheduler thread).  Planning beside the scheduler raised the saturated
-    // rate on the serving mix (at 320,000 req/s: 407 against 357 GB/s
-    // served) but the hand-over to a planner thread added ~0.07 ms to the
-    // median latency at 20,000 req/s, so it is off by default.  The plain
-    // baseline always plans inline.
-    int n_planners = 0;
+    // Planner threads (MTGP_SERVER_PLANNERS, default 2; 0 = plan on the
+    // scheduler thread).  With the classes taking turns for batch slots,
+    // two planner threads served the whole offered 368 GB/s of the serving
+    // mix at 240,000 req/s (p99 6.6-9 ms) against 311-315 GB/s inline, and
+    // 396-415 against 325-335 GB/s at 320,000 req/s, with the same median
+    // at 20,000 req/s (0.32 ms).  The plain baseline always plans inline.
+    int n_planners = 2;
     if (const char *e = getenv("MTGP_SERVER_PLANNERS"))
         n_planners = std::max(0, std::min(16, atoi(e)));
     if (s->plain)
