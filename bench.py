@@ -424,6 +424,20 @@ def jump_latency(M, reps=20):
         for _ in range(reps):
             noalloc(C.byref(g.g), js.encode())
         out["fast_jump_noalloc_ms"] = round((time.perf_counter() - t) * 1e3 / reps, 4)
+    # mtgp32_fill, the drop-in's bulk call (window up, a plan over the GPU,
+    # window back, synchronous): per-call time for a host array of 2^10 and
+    # 2^20 numbers and a device buffer of 2^20 and 2^24
+    fills = {}
+    for n, dev in ((1 << 10, False), (1 << 20, False), (1 << 20, True), (1 << 24, True)):
+        dst = M.DeviceBuffer(4 * n) if dev else np.empty(n, np.uint32)
+        g.fill(0, n, dst)
+        k = 5 if n >= (1 << 24) else reps
+        t = time.perf_counter()
+        for _ in range(k):
+            g.fill(0, n, dst)
+        fills["%s_2^%d_ms" % ("device" if dev else "host", n.bit_length() - 1)] = round(
+            (time.perf_counter() - t) * 1e3 / k, 4)
+    out["fill_ms"] = fills
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import RefLib
     if RefLib.available("fast", 32):
@@ -433,10 +447,18 @@ def jump_latency(M, reps=20):
         for _ in range(reps):
             r.jump(js)
         out["reference_cpu_fast_jump_ms"] = round((time.perf_counter() - t) * 1e3 / reps, 4)
+        r.fill(0, 1 << 20)
+        t = time.perf_counter()
+        for _ in range(5):
+            r.fill(0, 1 << 20)
+        out["reference_cpu_genrand_2^20_ms"] = round((time.perf_counter() - t) * 1e3 / 5, 4)
     out["what"] = ("one jump by 3^162 steps of MTGP32-11213 set 0 (2803-character string): fast_jump = the "
                    "drop-in on a host mtgp32_fast_t (window up, the jump split over 11 waves by chunk range and "
                    "their partial windows XORed by a second launch, window back, synchronous); "
-                   "jump_poly = x^(3^162) mod P on the host; reference = the reference library on one core")
+                   "jump_poly = x^(3^162) mod P on the host; reference = the reference library on one core; "
+                   "fill_ms = one mtgp32_fill of 2^k uint32 into a host array or a device buffer (window up, "
+                   "plan, launches, window back, synchronous); reference_cpu_genrand = the same count of "
+                   "mtgp32_genrand_uint32 on one core")
     return out
 
 
