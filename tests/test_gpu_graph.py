@@ -122,8 +122,12 @@ def test_two_phase_launch_serial_and_graph(mtgp, oracle):
 
 def test_tuning_knobs_and_jump_cost_grid(mtgp, oracle):
     """Tuning knobs round-trip and reject bad input; a plan whose jumps are
-    dear next to its cells (the jump-cost rule) uses fewer cells than the
-    two-size grid, and both grids give the oracle's numbers."""
+    dear next to its cells takes the jump-cost rule's grid (the grid that
+    two_size_rho = 0 forces, not the two-size grid that a huge
+    two_size_rho forces), and both grids give the oracle's numbers.  (Cell
+    counts depend on the default workgroups per CU, which an earlier
+    autotune in this process may have changed, so the grids are compared
+    with each other, not with fixed numbers.)"""
     assert mtgp.tuning_get("two_size_rho") > 0
     with pytest.raises(mtgp.MTGPError):
         mtgp.tuning_set("no_such_knob", 1)
@@ -134,8 +138,10 @@ def test_tuning_knobs_and_jump_cost_grid(mtgp, oracle):
     gs, reqs, want, bufs = _plan_case(mtgp, oracle, count=2000000)
     with mtgp.tuning(two_size_rho=1e9):
         two = gs.plan(reqs)
+    with mtgp.tuning(two_size_rho=0):
+        rule = gs.plan(reqs)
     cheap = gs.plan(reqs)
-    assert cheap.stats()["items"] < two.stats()["items"]
+    assert cheap.stats()["items"] == rule.stats()["items"] != two.stats()["items"]
     for plan in (two, cheap):
         for b, e in zip(bufs, want):
             b.from_host(np.full(e.nbytes, 0x5A, np.uint8))
