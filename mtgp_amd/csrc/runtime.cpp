@@ -447,6 +447,9 @@ class Pool {
             throw Error(err_);
     }
 
+    // Threads a call runs on (the workers and the caller).
+    int threads() const { return (int)workers_.size() + 1; }
+
   private:
     Pool() {
         const int hw = (int)std::thread::hardware_concurrency();
@@ -632,6 +635,7 @@ JumpCache::JumpCache(Context &c, const std::vector<int> &slots, uint64_t cell, i
         for (int k = 1; k < 48; k++)
             e.pow2.push_back(e.mod->sqrmod(e.pow2.back()));
         e.t1.resize(B_);
+        e.t1w.resize(B_);
         const gf2::Poly big = e.mod->xpow(std::vector<uint64_t>{B_ * cell});
         e.t2p.resize(kT2);
         e.t2p[0] = gf2::Poly{1};
@@ -646,7 +650,8 @@ JumpCache::JumpCache(Context &c, const std::vector<int> &slots, uint64_t cell, i
         for (uint64_t k = k0; k < k1; k++) {
             if (k > 0) {
                 cur = k == k0 ? cur : e.mod->mulmod(cur, e.pow2[0]);
-                e.t1[k] = jump_mask16(gf2::to_words32(cur, e.words32));
+                e.t1w[k] = gf2::to_words32(cur, e.words32);
+                e.t1[k] = jump_mask16(e.t1w[k]);
             }
         }
     });
@@ -666,6 +671,11 @@ std::shared_ptr<JumpCache> JumpCache::shared(Context &c, const std::vector<int> 
     auto it = g_caches.find(key);
     if (it != g_caches.end())
         return it->second;
+    // at most kMaxCaches kept: drop the ones nobody else holds
+    constexpr size_t kMaxCaches = 32;
+    if (g_caches.size() >= kMaxCaches)
+        for (auto e = g_caches.begin(); e != g_caches.end();)
+            e = e->second.use_count() == 1 ? g_caches.erase(e) : std::next(e);
     auto p = std::make_shared<JumpCache>(c, keys, cell, kmax);
     g_caches[key] = p;
     return p;
@@ -699,6 +709,15 @@ bool JumpCache::get_masks(int slot, uint64_t pos, std::vector<uint16_t> &m16, bo
     }
     m16 = jump_mask16(gf2::to_words32(acc, e.words32));
     return true;
+}
+
+const std::vector<uint32_t> *JumpCache::table_words(int slot, uint64_t pos) const {
+    if (pos == 0 || pos % cell_)
+        return nullptr;
+    auto it = entries_.find(slot);
+    if (it == entries_.end() || pos / cell_ >= B_)
+        return nullptr;
+    return &it->second.t1w[(size_t)(pos / cell_)];
 }
 
 bool JumpCache::get(int slot, uint64_t pos, std::vector<uint32_t> &words, bool cheap) const {
@@ -740,6 +759,14 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 }  // namespace
 
 double jump_cost_numbers(int mexp, int n) { return 0.026 * (double)mexp * (double)n; }
+
+// Plans on a uniform grid with a generator of at least kAutoCacheCells
+// cells take their jump polynomials from a process-wide JumpCache of
+// kAutoCacheCells * 2^i entries, up to kAutoCacheMaxK.
+constexpr int kAutoCacheCells = 64;
+constexpr int kAutoCacheMaxK = 2048;
+// Cells per block of an incremental polynomial chain on the host pool.
+constexpr size_t kChainMin = 32;
 
 GridModel grid_model_of(int bits, int cls, int mexp, int n) {
     GridModel m;
@@ -1016,11 +1043,38 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     };
     std::vector<std::vector<std::vector<uint32_t>>> polys((size_t)G);
     std::vector<std::vector<std::vector<uint16_t>>> m16((size_t)G);
+    // A uniform grid (one cell size, no caller cache) with a generator of
+    // many cells -- one long stream, as mtgp32_fill and the torch Generator
+    // make -- takes its polynomials x^(k S) from a process-wide cache: a
+    // repeated request of the same size then needs no polynomial product,
+    // and the first builds the table in blocks on the host pool.
+    std::shared_ptr<JumpCache> auto_cache;
+    const JumpCache *cache = opt.cache;
+    if (!cache && frac == 0 && S > 0) {
+        uint64_t kmax = 0;
+        std::vector<int> slots;
+        for (int g = 0; g < G; g++) {
+            const auto &cv = cells[(size_t)g];
+            if (cv.size() < (size_t)kAutoCacheCells)
+                continue;
+            slots.push_back(gens[(size_t)g].slot);
+            for (const Cell &cl : cv)
+                if (cl.start % S == 0)
+                    kmax = std::max<uint64_t>(kmax, cl.start / S);
+        }
+        if (!slots.empty() && kmax <= (uint64_t)kAutoCacheMaxK) {
+            int k = kAutoCacheCells;
+            while ((uint64_t)k < kmax)
+                k *= 2;
+            auto_cache = JumpCache::shared(c, slots, S, k);
+            cache = auto_cache.get();
+        }
+    }
     auto cached = [&](int g, size_t k, bool cheap) {
         const int slot = gens[(size_t)g].slot;
         const uint64_t pos = cells[(size_t)g][k].start;
-        return masks ? opt.cache->get_masks(slot, pos, m16[(size_t)g][k], cheap)
-                     : opt.cache->get(slot, pos, polys[(size_t)g][k], cheap);
+        return masks ? cache->get_masks(slot, pos, m16[(size_t)g][k], cheap)
+                     : cache->get(slot, pos, polys[(size_t)g][k], cheap);
     };
     std::vector<int> compute;                      // generators with cells off the cache
     std::vector<std::pair<int, size_t>> composed;  // cells the cache makes with one product
@@ -1032,9 +1086,9 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         for (size_t k = 0; k < cv.size(); k++) {
             if (cv[k].start == 0)
                 continue;
-            if (opt.cache && cached(g, k, true))
+            if (cache && cached(g, k, true))
                 continue;
-            if (opt.cache && cv[k].start % S == 0)
+            if (cache && cv[k].start % S == 0)
                 composed.emplace_back(g, k);
             else
                 need = true;
@@ -1045,8 +1099,39 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
     parallel_for((int)composed.size(), [&](int i) {
         (void)cached(composed[(size_t)i].first, composed[(size_t)i].second, false);
     });
-    parallel_for((int)compute.size(), [&](int i) {
-        const int g = compute[(size_t)i];
+    // A cell the cache could not make (its set is not in the cache, or the
+    // position is beyond it) is computed below with its generator's others.
+    for (const auto &gk : composed)
+        if (polys[(size_t)gk.first][gk.second].empty() && m16[(size_t)gk.first][gk.second].empty() &&
+            std::find(compute.begin(), compute.end(), gk.first) == compute.end())
+            compute.push_back(gk.first);
+    // The rest incrementally along each generator (x^b = x^a * x^(b - a)),
+    // in blocks of cells so that one long stream is spread over the host
+    // pool too: each block starts with its own x^start.
+    struct Chain {
+        int g;
+        std::vector<size_t> ks;
+    };
+    std::vector<Chain> chains;
+    {
+        const int want = std::max(1, (2 * Pool::get().threads()) / std::max(1, (int)compute.size()));
+        for (int g : compute) {
+            const auto &cv = cells[(size_t)g];
+            std::vector<size_t> ks;
+            for (size_t k = 0; k < cv.size(); k++)
+                if (cv[k].start > 0 && polys[(size_t)g][k].empty() && m16[(size_t)g][k].empty())
+                    ks.push_back(k);
+            // blocks of at least kChainMin cells: a block's first power costs
+            // about as much as that many products
+            const size_t per = std::max<size_t>(kChainMin, (ks.size() + (size_t)want - 1) / (size_t)want);
+            for (size_t i = 0; i < ks.size(); i += per)
+                chains.push_back(Chain{g, std::vector<size_t>(ks.begin() + (long)i,
+                                                              ks.begin() + (long)std::min(ks.size(), i + per))});
+        }
+    }
+    parallel_for((int)chains.size(), [&](int i) {
+        const Chain &ch = chains[(size_t)i];
+        const int g = ch.g;
         const auto &cv = cells[(size_t)g];
         auto &out = polys[(size_t)g];
         const GenRef &gr = gens[(size_t)g];
@@ -1055,10 +1140,8 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
         std::map<uint64_t, gf2::Poly> step_cache;
         gf2::Poly cur;
         uint64_t cur_off = 0;
-        for (size_t k = 0; k < cv.size(); k++) {
+        for (size_t k : ch.ks) {
             const uint64_t off = cv[k].start;
-            if (off == 0 || !out[k].empty() || !m16[(size_t)g][k].empty())
-                continue;
             if (cur_off == 0) {
                 cur = mod->xpow(off);
             } else {
@@ -1204,14 +1287,30 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             gr.coef = (uint64_t)coef_host.size() * 4;  // fixed up below
             coef_host.resize(coef_host.size() + (size_t)wc * (size_t)njp, 0u);
             uint32_t *cf = coef_host.data() + gr.coef / 4;
-            for (int b = 0; b < nj; b++) {
+            // jump b's words in column b: the cell's own words, the cache's
+            // table entry, or converted from the mask form (on the host
+            // pool); then the array row by row
+            std::vector<const std::vector<uint32_t> *> wv((size_t)nj);
+            std::vector<std::vector<uint32_t>> conv((size_t)nj);
+            parallel_for(nj, [&](int b) {
                 const size_t k = m.cells[(size_t)b];
-                const std::vector<uint32_t> &pw32 = polys[(size_t)m.g][k];
-                const std::vector<uint32_t> w =
-                    pw32.empty() ? words_from_mask16(m16[(size_t)m.g][k], (ref.mexp + 31) / 32) : pw32;
-                for (size_t x = 0; x < w.size() && x < (size_t)wc; x++)
-                    cf[x * (size_t)njp + (size_t)b] = w[x];
-            }
+                const std::vector<uint32_t> *w = &polys[(size_t)m.g][k];
+                if (w->empty() && cache)
+                    if (const auto *tw = cache->table_words(ref.slot, cells[(size_t)m.g][k].start))
+                        w = tw;
+                if (w->empty()) {
+                    conv[(size_t)b] = words_from_mask16(m16[(size_t)m.g][k], (ref.mexp + 31) / 32);
+                    w = &conv[(size_t)b];
+                }
+                wv[(size_t)b] = w;
+            });
+            parallel_for((wc + 15) / 16, [&](int blk) {
+                for (int x = 16 * blk; x < std::min(wc, 16 * blk + 16); x++) {
+                    uint32_t *row = cf + (size_t)x * (size_t)njp;
+                    for (int b = 0; b < nj; b++)
+                        row[b] = (size_t)x < wv[(size_t)b]->size() ? (*wv[(size_t)b])[(size_t)x] : 0u;
+                }
+            });
             gr.nj = nj;
             gr.njp = njp;
             gr.kpad = kpad;
@@ -1865,40 +1964,32 @@ static int fill_impl(Gen *g, int kind, void *out, size_t n, int bits) {
     }
     if (n && !out)
         throw Error("NULL output buffer");
-    char *d_states = nullptr;
-    void *d_out = nullptr;
-    auto cleanup = [&] {
-        (void)hipFree(d_states);
-        if (!dev_out)
-            (void)hipFree(d_out);
-    };
-    try {
-        hip_check(hipMalloc((void **)&d_states, 2 * (size_t)N * sizeof(Word)), "hipMalloc");
-        hip_check(hipMemcpy(d_states, w.data(), (size_t)N * sizeof(Word), hipMemcpyHostToDevice), "hipMemcpy");
-        if (dev_out)
-            d_out = out;
-        else if (n)
-            hip_check(hipMalloc(&d_out, n * eb), "hipMalloc");
-        GenRef r;
-        r.bits = bits;
-        r.mexp = g->params.mexp;
-        r.n = N;
-        r.slot = slot;
-        r.state_in = d_states;
-        r.state_out = d_states + (size_t)N * sizeof(Word);
-        mtgp_request_t q{0, kind, (uint64_t)n, 0, d_out};
-        Plan plan;
-        plan.build(c, {r}, {q}, 0);
-        plan.launch(0);
-        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-        hip_check(hipMemcpy(w.data(), r.state_out, (size_t)N * sizeof(Word), hipMemcpyDeviceToHost), "hipMemcpy");
-        if (n && !dev_out)
-            hip_check(hipMemcpy(out, d_out, n * eb, hipMemcpyDeviceToHost), "hipMemcpy");
-    } catch (...) {
-        cleanup();
-        throw;
+    std::lock_guard<std::mutex> fill_lock(c.fill_mu);
+    if (!c.fill_plan)
+        c.fill_plan = new Plan();
+    Plan &plan = *c.fill_plan;
+    grow_device(c.fill_states, c.fill_states_cap, 2 * (size_t)N * sizeof(Word), "hipMalloc fill states");
+    char *d_states = c.fill_states;
+    void *d_out = out;
+    if (n && !dev_out) {
+        grow_device(c.fill_out, c.fill_out_cap, n * eb, "hipMalloc fill output");
+        d_out = c.fill_out;
     }
-    cleanup();
+    hip_check(hipMemcpy(d_states, w.data(), (size_t)N * sizeof(Word), hipMemcpyHostToDevice), "hipMemcpy");
+    GenRef r;
+    r.bits = bits;
+    r.mexp = g->params.mexp;
+    r.n = N;
+    r.slot = slot;
+    r.state_in = d_states;
+    r.state_out = d_states + (size_t)N * sizeof(Word);
+    mtgp_request_t q{0, kind, (uint64_t)n, 0, d_out};
+    plan.build(c, {r}, {q}, 0);
+    plan.launch(0);
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    hip_check(hipMemcpy(w.data(), r.state_out, (size_t)N * sizeof(Word), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (n && !dev_out)
+        hip_check(hipMemcpy(out, d_out, n * eb, hipMemcpyDeviceToHost), "hipMemcpy");
     memcpy(g->status->array, w.data(), (size_t)N * sizeof(Word));
     g->status->idx = N - 1;
     return 0;

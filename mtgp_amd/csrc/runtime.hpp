@@ -131,6 +131,15 @@ struct Context {
     // mtgp_tuning_set and the autotuner write them under an exclusive one.
     mutable std::shared_mutex tune_mu;
 
+    // mtgp32_fill / mtgp64_fill reuse one plan and grow-only device
+    // buffers (start and end windows, an output staging buffer for host
+    // destinations), so repeated calls allocate nothing; calls on one
+    // device serialise on fill_mu.  Kept until the process ends.
+    std::mutex fill_mu;
+    struct Plan *fill_plan = nullptr;
+    char *fill_states = nullptr, *fill_out = nullptr;
+    size_t fill_states_cap = 0, fill_out_cap = 0;
+
     std::map<std::pair<const void *, int>, size_t> reserve_cache;
     // Dynamic LDS that limits `fn` to `per_cu` resident workgroups per CU
     // (0 when the kernel's own limit is already at most per_cu).
@@ -230,12 +239,17 @@ class JumpCache {
     bool get_masks(int slot, uint64_t pos, std::vector<uint16_t> &m16, bool cheap = false) const;
     // The same as coefficient words.
     bool get(int slot, uint64_t pos, std::vector<uint32_t> &words, bool cheap = false) const;
+    // The table entry x^pos mod P_slot as coefficient words, if pos is one
+    // (a positive multiple of the cell size below the first table's end);
+    // nullptr otherwise.
+    const std::vector<uint32_t> *table_words(int slot, uint64_t pos) const;
 
   private:
     struct Entry {
         std::shared_ptr<gf2::Modulus> mod;
         int words32 = 0;
         std::vector<std::vector<uint16_t>> t1;  // [k] = x^(k cell) in mask form, k < B
+        std::vector<std::vector<uint32_t>> t1w; // the same as coefficient words
         std::vector<gf2::Poly> t2p;              // [j] = x^(j B cell)
         std::vector<gf2::Poly> pow2;             // [i] = x^(2^i cell)
     };
