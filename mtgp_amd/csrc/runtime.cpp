@@ -2656,12 +2656,14 @@ int mtgp_plan_launch_info(const mtgp_plan_t *p, int k, int *bits, int *cls, int 
 
 int mtgp_genset_generate(mtgp_genset_t *s, const mtgp_request_t *reqs, int nreq, void *stream) {
     API_BEGIN
-    mtgp_plan p;
-    p.set = s;
-    p.plan.build(*s->ctx, s->refs(), std::vector<mtgp_request_t>(reqs, reqs + nreq), 0);
-    p.plan.launch((hipStream_t)stream);
+    std::lock_guard<std::mutex> lk(s->gen_mu);
+    if (!s->gen_plan)
+        s->gen_plan = std::make_unique<Plan>();
+    Plan &plan = *s->gen_plan;
+    plan.build(*s->ctx, s->refs(), std::vector<mtgp_request_t>(reqs, reqs + nreq), 0);
+    plan.launch((hipStream_t)stream);
     hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
-    for (int g : p.plan.commits)
+    for (int g : plan.commits)
         s->set_cur(g, s->cur[(size_t)g] ^ 1);
     return 0;
     API_END(-1)

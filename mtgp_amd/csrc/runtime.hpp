@@ -448,7 +448,12 @@ struct mtgp_genset {
         int combine_first = 0, combine_count = 0;
         size_t o_items = 0, o_queue = 0;
     } sub;
+    // mtgp_genset_generate's plan, reused so that repeated calls allocate
+    // nothing once their sizes have been seen (calls on one set serialise).
+    std::mutex gen_mu;
+    std::unique_ptr<mtgp::Plan> gen_plan;
     ~mtgp_genset() {
+        gen_plan.reset();
         if (d_states)
             (void)hipFree(d_states);
         if (sub.d)
