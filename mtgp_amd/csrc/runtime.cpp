@@ -2695,7 +2695,120 @@ int mtgp_genset_substreams(mtgp_genset_t *s, const void *base, const char *hex) 
     const size_t stride = align256(wbytes);
     const int occ = c.jump_blocks_per_cu[bits == 64][cls];
     auto &W = s->sub;
-    if (!W.d || W.key != hex) {
+    // Many substreams of an MTGP32 set: every one is a jump from the base,
+    // so all of them are one matrix-core group (J^t for t < G from the
+    // host, once per jump string).
+    const bool matrix = bits == 32 && c.jump_mfma_min > 0 && G - 1 >= c.jump_mfma_min;
+    if (matrix && (!W.d || W.key != hex || !W.matrix)) {
+        gf2::Poly J;
+        if (!gf2::from_hex(hex, J))
+            throw Error("jump string contains a non-hex character");
+        auto mod = modulus_for(c, s->slot[0]);
+        J = mod->reduce_any(J);
+        const int nj = G - 1, njp = (nj + 31) / 32 * 32;
+        const int kpad = mj_kpad(s->mexp[0]), wc = kpad / 32, pw = mj_pw(kpad, s->n[0]);
+        const int pwords = (s->mexp[0] + 31) / 32;
+        // coef[w * njp + (t - 1)] = word w of J^t, chains of J products in
+        // blocks on the host pool (a block starts from J^t0 by squarings)
+        std::vector<uint32_t> coef((size_t)wc * (size_t)njp, 0u);
+        constexpr int kBlock = 128;
+        parallel_for((nj + kBlock - 1) / kBlock, [&](int blk) {
+            const int t0 = 1 + blk * kBlock, t1 = std::min(G, t0 + kBlock);
+            gf2::Poly cur, sq = J;
+            for (int e = t0; e; e >>= 1) {
+                if (e & 1)
+                    cur = cur.empty() ? sq : mod->mulmod(cur, sq);
+                if (e >> 1)
+                    sq = mod->sqrmod(sq);
+            }
+            for (int t = t0; t < t1; t++) {
+                if (t > t0)
+                    cur = mod->mulmod(cur, J);
+                const std::vector<uint32_t> w = gf2::to_words32(cur, pwords);
+                for (size_t x = 0; x < w.size() && x < (size_t)wc; x++)
+                    coef[x * (size_t)njp + (size_t)(t - 1)] = w[x];
+            }
+        });
+        // workspace: [base | group | items | coef | planes | windows 1..G-1 | combine items]
+        std::vector<MJumpItem> mit[kMjRowTiles + 1];
+        for (int r0 = 0; r0 < nj; r0 += 32 * kMjRowTiles) {
+            const int rt = std::min(kMjRowTiles, (nj - r0 + 31) / 32);
+            for (int j0 = 0; j0 < s->n[0]; j0 += mj_words(rt))
+                mit[rt].push_back(MJumpItem{0, r0, j0, 0});
+        }
+        std::vector<MJumpItem> items;
+        for (int rt = 1; rt <= kMjRowTiles; rt++) {
+            W.mi_first[rt] = (int)items.size();
+            W.mi_count[rt] = (int)mit[rt].size();
+            items.insert(items.end(), mit[rt].begin(), mit[rt].end());
+        }
+        W.o_mgroup = stride;
+        W.o_mitems = align256(W.o_mgroup + sizeof(MJumpGroup));
+        const size_t o_coef = align256(W.o_mitems + items.size() * sizeof(MJumpItem));
+        const size_t o_planes = align256(o_coef + coef.size() * 4);
+        const size_t o_out = align256(o_planes + (size_t)pw * 32 * 4);
+        W.o_items = align256(o_out + (size_t)nj * wbytes);
+        const size_t need = W.o_items + (size_t)G * sizeof(JumpItem);
+        W.key.clear();
+        if (need > W.cap) {
+            if (W.d)
+                hip_check(hipFree(W.d), "hipFree");
+            W.d = nullptr;
+            W.cap = 0;
+            hip_check(hipMalloc((void **)&W.d, need), "hipMalloc substream workspace");
+            W.cap = need;
+        }
+        const uint64_t dw = (uint64_t)(uintptr_t)W.d;
+        MJumpGroup gr{};
+        gr.state_in = dw;
+        gr.in_parts = 1;
+        gr.planes = dw + o_planes;
+        gr.coef = dw + o_coef;
+        gr.out = dw + o_out;
+        gr.out_stride = wbytes;
+        gr.nj = nj;
+        gr.njp = njp;
+        gr.kpad = kpad;
+        gr.pw = pw;
+        gr.param = s->slot[0];
+        gr.n = s->n[0];
+        // combine: substream t's window (the base for t = 0) into slot 0
+        std::vector<JumpItem> copies((size_t)G);
+        for (int t = 0; t < G; t++) {
+            JumpItem &ji = copies[(size_t)t];
+            ji.state_in = t ? dw + o_out + (uint64_t)(t - 1) * wbytes : dw;
+            ji.in_parts = 1;
+            ji.states_out = (uint64_t)(uintptr_t)s->state(t, 0);
+            ji.state_bytes = (int32_t)wbytes;
+        }
+        W.combine_first = 0;
+        W.combine_count = G;
+        hip_check(hipMemcpy(W.d + W.o_mgroup, &gr, sizeof gr, hipMemcpyHostToDevice), "hipMemcpy group");
+        hip_check(hipMemcpy(W.d + W.o_mitems, items.data(), items.size() * sizeof(MJumpItem), hipMemcpyHostToDevice),
+                  "hipMemcpy items");
+        hip_check(hipMemcpy(W.d + o_coef, coef.data(), coef.size() * 4, hipMemcpyHostToDevice), "hipMemcpy coef");
+        hip_check(hipMemcpy(W.d + W.o_items, copies.data(), copies.size() * sizeof(JumpItem), hipMemcpyHostToDevice),
+                  "hipMemcpy combine items");
+        W.first.clear();
+        W.matrix = true;
+        W.key = hex;
+    }
+    if (matrix) {
+        hip_check(hipMemcpy(W.d, base, wbytes, hipMemcpyHostToDevice), "hipMemcpy base");
+        const MJumpGroup *d_group = reinterpret_cast<const MJumpGroup *>(W.d + W.o_mgroup);
+        const MJumpItem *d_mitems = reinterpret_cast<const MJumpItem *>(W.d + W.o_mitems);
+        hip_check(launch_planes32(d_group, 1, c.d_params, 0), "planes kernel launch");
+        for (int rt = 1; rt <= kMjRowTiles; rt++)
+            hip_check(launch_jump_mfma32(d_group, d_mitems + W.mi_first[rt], W.mi_count[rt], rt, 0),
+                      "matrix-core jump launch");
+        hip_check(launch_combine(reinterpret_cast<const JumpItem *>(W.d + W.o_items), G, 0), "combine launch");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        for (int g = 0; g < G; g++)
+            s->set_cur(g, 0);
+        return 0;
+    }
+    if (!W.d || W.key != hex || W.matrix) {
+        W.matrix = false;
         // Substream t = J^t applied to the base, built on the device in
         // radix-4 levels: level m makes t = a 4^m + j (a = 1..3, j < 4^m)
         // as J^(a 4^m) applied to substream j -- the decomposition of t in

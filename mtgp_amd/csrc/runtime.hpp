@@ -447,6 +447,12 @@ struct mtgp_genset {
         std::vector<int> first;  // jump launches: items [first[k], first[k+1])
         int combine_first = 0, combine_count = 0;
         size_t o_items = 0, o_queue = 0;
+        // Matrix-core variant: one MJumpGroup of every substream from the
+        // base, its items by row-tile count [mi_first[rt], mi_first[rt] +
+        // mi_count[rt]) at o_mitems.
+        bool matrix = false;
+        size_t o_mgroup = 0, o_mitems = 0;
+        int mi_first[mtgp::kMjRowTiles + 1] = {}, mi_count[mtgp::kMjRowTiles + 1] = {};
     } sub;
     // mtgp_genset_generate's plan, reused so that repeated calls allocate
     // nothing once their sizes have been seen (calls on one set serialise).
