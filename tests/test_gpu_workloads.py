@@ -168,10 +168,14 @@ def test_substreams_3_162(mtgp, oracle):
         assert (got == want).all(), i
 
 
-def test_substreams_2_64(mtgp, oracle):
+@pytest.mark.parametrize("mfma_min", [None, 0])
+def test_substreams_2_64(mtgp, oracle, mfma_min):
     """Substream i = set 0, seed 777, jumped by i * 2^64 steps (a host-made
     polynomial), 2^20 uint32 each: samples against the oracle's direct
-    x^(i 2^64) mod P jump."""
+    x^(i 2^64) mod P jump -- start states from the matrix cores (default
+    for 4096 substreams) and from the radix-4 levels of batched jumps
+    (matrix cores off)."""
+    print("substreams 2^64, jump_mfma_min %s" % mfma_min, flush=True)
     count = 1 << 20
     p = oracle.params(32, 11213, 0)
     base = oracle.init_state(p, 777)
@@ -179,7 +183,11 @@ def test_substreams_2_64(mtgp, oracle):
     words = 11213 // 64 + 2
     P = oracle.hex_to_poly(mtgp.charpoly_hex(32, 11213, 0), words)
     assert oracle.poly_to_hex(oracle.xpow_mod(1 << 64, P)) == js
-    buf, _ = _substreams(mtgp, base, js, count)
+    if mfma_min is None:
+        buf, _ = _substreams(mtgp, base, js, count)
+    else:
+        with mtgp.tuning(jump_mfma_min=mfma_min):
+            buf, _ = _substreams(mtgp, base, js, count)
     for i in (0, 1, 2, 3, 100, 2047, 4095):
         w = oracle.jump(p, base, oracle.poly_to_hex(oracle.xpow_mod(i << 64, P))) if i else base
         want, _ = oracle.generate(p, w, 0, 1 << 16)
