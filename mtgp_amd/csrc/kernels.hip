@@ -1565,6 +1565,8 @@ __global__ __launch_bounds__(64) void mtgp32_jump_mfma_kernel(const MJumpGroup *
     const MJumpGroup &G = groups[I.group];
     const int L = threadIdx.x, col = L & 31, kg = L >> 5;
     const int nj = G.nj, njp = G.njp, r0 = I.r0, j0 = I.j0;
+    if (r0 < 0 || r0 >= nj || j0 < 0 || j0 >= G.n)
+        return;  // not an item of this group (the host makes none)
     const gu32 *T = reinterpret_cast<const gu32 *>(G.planes);
     const gu32 *C = reinterpret_cast<const gu32 *>(G.coef);
     const uint32_t sh = (uint32_t)j0 & 31u;
