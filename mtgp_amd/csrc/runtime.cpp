@@ -612,7 +612,7 @@ gf2::Poly poly_of_words32(const std::vector<uint32_t> &w) {
 }
 }  // namespace
 
-JumpCache::JumpCache(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax) : cell_(cell) {
+JumpCache::JumpCache(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax, bool far) : cell_(cell) {
     if (cell == 0 || kmax < 1)
         throw Error("jump cache needs a cell size and kmax >= 1");
     B_ = (uint64_t)kmax + 1;
@@ -632,10 +632,12 @@ JumpCache::JumpCache(Context &c, const std::vector<int> &slots, uint64_t cell, i
         e.mod = modulus_for(c, keys[(size_t)i]);
         e.words32 = (int)((e.mod->deg() + 31) / 32);
         e.pow2.push_back(e.mod->xpow(cell));
-        for (int k = 1; k < 48; k++)
-            e.pow2.push_back(e.mod->sqrmod(e.pow2.back()));
         e.t1.resize(B_);
         e.t1w.resize(B_);
+        if (!far)
+            return;
+        for (int k = 1; k < 48; k++)
+            e.pow2.push_back(e.mod->sqrmod(e.pow2.back()));
         const gf2::Poly big = e.mod->xpow(std::vector<uint64_t>{B_ * cell});
         e.t2p.resize(kT2);
         e.t2p[0] = gf2::Poly{1};
@@ -662,11 +664,12 @@ std::mutex g_cache_mu;
 std::map<std::tuple<int, uint64_t, int, std::vector<int>>, std::shared_ptr<JumpCache>> g_caches;
 }  // namespace
 
-std::shared_ptr<JumpCache> JumpCache::shared(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax) {
+std::shared_ptr<JumpCache> JumpCache::shared(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax,
+                                             bool far) {
     std::vector<int> keys(slots);
     std::sort(keys.begin(), keys.end());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-    auto key = std::make_tuple(c.device, cell, kmax, keys);
+    auto key = std::make_tuple(c.device, cell, far ? kmax : -kmax, keys);
     std::lock_guard<std::mutex> lk(g_cache_mu);
     auto it = g_caches.find(key);
     if (it != g_caches.end())
@@ -676,7 +679,7 @@ std::shared_ptr<JumpCache> JumpCache::shared(Context &c, const std::vector<int> 
     if (g_caches.size() >= kMaxCaches)
         for (auto e = g_caches.begin(); e != g_caches.end();)
             e = e->second.use_count() == 1 ? g_caches.erase(e) : std::next(e);
-    auto p = std::make_shared<JumpCache>(c, keys, cell, kmax);
+    auto p = std::make_shared<JumpCache>(c, keys, cell, kmax, far);
     g_caches[key] = p;
     return p;
 }
@@ -1066,7 +1069,7 @@ void Plan::build(Context &c, const std::vector<GenRef> &gens, const std::vector<
             int k = kAutoCacheCells;
             while ((uint64_t)k < kmax)
                 k *= 2;
-            auto_cache = JumpCache::shared(c, slots, S, k);
+            auto_cache = JumpCache::shared(c, slots, S, k, false);
             cache = auto_cache.get();
         }
     }

@@ -229,9 +229,12 @@ bool auto_grid(const Context &c, uint64_t total_numbers, int n_requests, int n_g
 // it.
 class JumpCache {
   public:
-    JumpCache(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax);
+    // far = false leaves out the tables for positions beyond kmax cells
+    // (lookups there then fail and the planner computes those cells).
+    JumpCache(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax, bool far = true);
     // One cache per (device, cell, kmax, slots) for the whole process.
-    static std::shared_ptr<JumpCache> shared(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax);
+    static std::shared_ptr<JumpCache> shared(Context &c, const std::vector<int> &slots, uint64_t cell, int kmax,
+                                             bool far = true);
     uint64_t cell() const { return cell_; }
     // x^pos mod P_slot in the per-residue mask form (jump_mask16), when pos
     // is a positive multiple of the cell size and the slot is cached; false
