@@ -388,9 +388,10 @@ def run_configs(M):
     d = line(S * 4 * count, jump_ms, gen_ms, plan.stats()["items"], substream_setup_ms=round(setup_ms, 3),
              substream_setup_first_ms=round(first_ms, 3),
              workload="4096 substreams i*3^162 of MTGP32-11213 set 0 (seed 777), 2^20 uint32 each, end to end: "
-                      "substream start states made on the device in radix-4 levels (6 batched jump launches of "
-                      "J^(a 4^m) + one combine launch; host time included, wall clock, median of 3 calls after the "
-                      "first, which also builds the work list: substream_setup_first_ms) + generation")
+                      "substream start states made on the device as one matrix-core group (the 4095 states "
+                      "J^(i 3^162) w0 as fp4 products of the coefficient rows with the base window's bit planes; "
+                      "host time included, wall clock, median of 3 calls after the first, which also builds the "
+                      "coefficient rows: substream_setup_first_ms) + generation")
     d["ms"] = round(setup_ms + jump_ms + gen_ms, 4)
     d["gbs"] = round(S * 4 * count / (d["ms"] * 1e-3) / 1e9, 2)
     out["4"] = d

@@ -17,6 +17,12 @@ inline void store_u32(store_window_t w, uint32_t off, uint32_t v) {
     if (off < w.bytes)
         memcpy(reinterpret_cast<char *>(w.base) + off, &v, 4);
 }
+inline void store_u64(store_window_t w, uint32_t off, uint32_t lo, uint32_t hi) {
+    if (off < w.bytes) {
+        memcpy(reinterpret_cast<char *>(w.base) + off, &lo, 4);
+        memcpy(reinterpret_cast<char *>(w.base) + off + 4, &hi, 4);
+    }
+}
 
 typedef int mfma_v8i __attribute__((ext_vector_type(8)));
 typedef float mfma_v16f __attribute__((ext_vector_type(16)));
