@@ -73,10 +73,13 @@ struct Context {
     // generator's positions in cells sized for one round over all slots, the
     // rest in cells sized for a second round (so the last round's cells are
     // small and fill the tail); 0 = equal cells, segment_rounds rounds.
-    // 0.8 measured best on the batched MTGP32-11213 workload at 12
-    // workgroups per CU (4.38-4.41 ms per step; 0.7: 4.43-4.52, 0.85:
-    // 4.55, 0.9: 4.71; equal cells at 1.5 rounds were slower still).
-    double segment_split = 0.8;
+    // 0.76 measured best on the batched MTGP32-11213 workload at 12
+    // workgroups per CU with device-scope output stores (jump + generation
+    // 4.19 ms; 0.75: 4.19-4.23, 0.77: 4.21-4.23, 0.74 and 0.78: 4.26-4.30,
+    // 0.8: 4.30-4.35, 0.7: 4.32-4.34, 0.85: 4.47-4.52); MTGP64-DC batches
+    // gain 1.5% over 0.8 and the other batches are even.  (With wave-scope
+    // stores 0.8 had been best.)
+    double segment_split = 0.76;
     // Small cells of the two-size grid per resident slot (their count is
     // about small_rounds * slots; each costs a jump).
     double small_rounds = 1.0;
