@@ -107,12 +107,13 @@ Context &context() {
         }
     if (const char *e = getenv("MTGP_BLOCKS_PER_CU")) {
         const int v = atoi(e);
-        if (v > 0)
+        if (v > 0) {
             c->default_bpc = v;
+            for (int b = 0; b < 2; b++)
+                for (int cls = 0; cls < kClasses; cls++)
+                    c->bpc[b][cls] = v;
+        }
     }
-    for (int b = 0; b < 2; b++)
-        for (int cls = 0; cls < kClasses; cls++)
-            c->bpc[b][cls] = c->default_bpc;
     if (const char *e = getenv("MTGP_SEGMENT_SPLIT")) {
         const double v = atof(e);
         if (v >= 0 && v < 1)

@@ -61,8 +61,13 @@ struct Context {
     // against 4.60-4.63 ms at 16 and 4.71 ms at 8.
     int default_bpc = 12;
     // Resident workgroups per CU for generation launches per [bits==64][class]
-    // (default_bpc until the autotuner measures better).
-    int bpc[2][4] = {{12, 12, 12, 12}, {12, 12, 12, 12}};
+    // (until the autotuner measures better; MTGP_BLOCKS_PER_CU sets them
+    // all).  default_bpc everywhere but the MTGP64 128-word-step class
+    // (dynamic-creator sets, N = 176): 8, which halves the cells' jumps
+    // there; its 64-set batch measured 4207-4212 GB/s per step at 8
+    // against 4032-4041 at 12 and 3557-3574 at 16, while the other classes
+    // are even or slower at 8.
+    int bpc[2][4] = {{12, 12, 12, 12}, {12, 12, 12, 8}};
     // Automatic segmentation cuts the work of a plan into about this many
     // segments per resident workgroup slot: more rounds balance the tail
     // better, each extra segment costs one jump-ahead (1.5 measured best for
