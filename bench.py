@@ -655,8 +655,10 @@ def measured_traffic(path=os.path.join(ROOT, "profiles", "mtgp32_items_batched11
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    # the generation launch's clock settles over its first ~8 launches (the
+    # board's power management), so the default warm-up covers them
+    ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--segment", type=int, default=0, help="numbers per segment (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
