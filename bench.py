@@ -272,10 +272,11 @@ def run_sustained(M, gs, plan_a, buf_a, reqs_a, seeds, lengths, stream, cycles, 
                    "batch written against the oracle jumped over the earlier batches"}
 
 
-def _time_plan(M, plan, stream, reps=3):
-    """(jump_ms, generate_ms) of a plan: mean over `reps` launches after one
-    warm-up, HIP events on the stream the kernels run on."""
-    plan.launch(stream)
+def _time_plan(M, plan, stream, reps=5, warmup=2):
+    """(jump_ms, generate_ms) of a plan: mean over `reps` launches after
+    `warmup` ones, HIP events on the stream the kernels run on."""
+    for _ in range(warmup):
+        plan.launch(stream)
     stream.synchronize()
     ev = [M.Event() for _ in range(3)]
     j = g = 0.0
