@@ -228,15 +228,21 @@ def run_sustained(M, gs, plan_a, buf_a, reqs_a, seeds, lengths, stream, cycles, 
     stream.synchronize()
     ev0, ev1 = M.Event(), M.Event()
     host = []
-    ev0.record(stream)
-    for k in range(cycles):
+    # `warm` untimed cycles first (fresh batches too): the GPU idled while
+    # the second plan was built, and its clock settles over the first
+    # launches after that
+    warm = 4
+    for k in range(warm + cycles):
+        if k == warm:
+            ev0.record(stream)
         p = plans[k % 2]
         t = time.perf_counter()
         if k:
             p.rebase(stream)
         p.launch(stream)
         p.commit()
-        host.append((time.perf_counter() - t) * 1e3)
+        if k >= warm:
+            host.append((time.perf_counter() - t) * 1e3)
     ev1.record(stream)
     stream.synchronize()
     ms = ev0.elapsed_ms(ev1)
@@ -244,7 +250,7 @@ def run_sustained(M, gs, plan_a, buf_a, reqs_a, seeds, lengths, stream, cycles, 
     # behind, for the GPU to free a plan's staging copy)
     work = []
     for k in range(4):
-        p = plans[(cycles + k) % 2]
+        p = plans[(warm + cycles + k) % 2]
         stream.synchronize()
         t = time.perf_counter()
         p.rebase(stream)
@@ -252,7 +258,7 @@ def run_sustained(M, gs, plan_a, buf_a, reqs_a, seeds, lengths, stream, cycles, 
         p.commit()
         work.append((time.perf_counter() - t) * 1e3)
     stream.synchronize()
-    cycles_run = cycles + 4
+    cycles_run = warm + cycles + 4
     exact = None
     if check:
         # the last cycle's batch (the cycles-th batch of every stream) in
@@ -266,7 +272,8 @@ def run_sustained(M, gs, plan_a, buf_a, reqs_a, seeds, lengths, stream, cycles, 
             "fresh_plan_ms": round(fresh_ms, 1), "bit_exact": exact,
             "how": "cycle k: plan k%2 rebased onto the states cycle k-1 committed (its work items re-pointed, "
                    "no polynomial work), launched, committed; enqueued back to back on one stream over two "
-                   "output buffers; host_ms_per_cycle includes waiting for the GPU (the host runs at most two "
+                   "output buffers, timed after 4 such cycles untimed; host_ms_per_cycle includes waiting for "
+                   "the GPU (the host runs at most two "
                    "cycles ahead), host_work_ms_per_cycle is one cycle's host calls with the GPU idle; "
                    "fresh_plan_ms is a second mtgp_plan_create of the same shape; bit_exact checks the last "
                    "batch written against the oracle jumped over the earlier batches"}
