@@ -1798,6 +1798,7 @@ float Plan::autotune(hipStream_t s, int reps, int *best_layout, int *best_bpc) {
     (void)hipEventDestroy(e1);
     configure(bl, bb);
     std::unique_lock<std::shared_mutex> tune_lock(ctx->tune_mu);
+    ctx->tune_epoch++;
     for (const auto &L : launches) {
         if (L.layout == bl)
             (one_round_grid ? ctx->layout_one_round : ctx->layout)[L.bits == 64][L.cls] = bl;
@@ -1987,8 +1988,23 @@ static int fill_impl(Gen *g, int kind, void *out, size_t n, int bits) {
     r.slot = slot;
     r.state_in = d_states;
     r.state_out = d_states + (size_t)N * sizeof(Word);
-    mtgp_request_t q{0, kind, (uint64_t)n, 0, d_out};
-    plan.build(c, {r}, {q}, 0);
+    Context::FillShape shape;
+    shape.slot = slot;
+    shape.bits = bits;
+    shape.kind = kind;
+    shape.n = n;
+    shape.out = d_out;
+    shape.states = d_states;
+    {
+        std::shared_lock<std::shared_mutex> tune_lock(c.tune_mu);
+        shape.epoch = c.tune_epoch;
+    }
+    if (!(shape == c.fill_shape)) {
+        c.fill_shape = Context::FillShape();  // a failed build leaves no reusable shape
+        mtgp_request_t q{0, kind, (uint64_t)n, 0, d_out};
+        plan.build(c, {r}, {q}, 0);
+        c.fill_shape = shape;
+    }
     plan.launch(0);
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     hip_check(hipMemcpy(w.data(), r.state_out, (size_t)N * sizeof(Word), hipMemcpyDeviceToHost), "hipMemcpy");
@@ -2101,6 +2117,7 @@ int mtgp_tuning_set(const char *name, double value) {
     Context &c = context();
     const std::string n = name ? name : "";
     std::unique_lock<std::shared_mutex> lk(c.tune_mu);
+    c.tune_epoch++;
     if (n == "overlap") {
         c.overlap = value != 0;
         return 0;
@@ -2661,10 +2678,36 @@ int mtgp_plan_launch_info(const mtgp_plan_t *p, int k, int *bits, int *cls, int 
 int mtgp_genset_generate(mtgp_genset_t *s, const mtgp_request_t *reqs, int nreq, void *stream) {
     API_BEGIN
     std::lock_guard<std::mutex> lk(s->gen_mu);
+    if (nreq < 0 || (nreq > 0 && !reqs))
+        throw Error("bad request list");
     if (!s->gen_plan)
         s->gen_plan = std::make_unique<Plan>();
     Plan &plan = *s->gen_plan;
-    plan.build(*s->ctx, s->refs(), std::vector<mtgp_request_t>(reqs, reqs + nreq), 0);
+    std::vector<mtgp_request_t> rv(reqs, reqs + nreq);
+    uint64_t epoch;
+    {
+        std::shared_lock<std::shared_mutex> tune_lock(s->ctx->tune_mu);
+        epoch = s->ctx->tune_epoch;
+    }
+    auto same = [&]() {
+        if (rv.empty() || epoch != s->gen_epoch || rv.size() != s->gen_reqs.size())
+            return false;
+        for (size_t i = 0; i < rv.size(); i++) {
+            const mtgp_request_t &a = rv[i], &b = s->gen_reqs[i];
+            if (a.gen != b.gen || a.kind != b.kind || a.count != b.count || a.skip != b.skip || a.out != b.out)
+                return false;
+        }
+        return true;
+    };
+    if (same()) {
+        plan.retarget(s->refs(), (hipStream_t)stream);
+    } else {
+        s->gen_reqs.clear();  // a failed build leaves nothing to reuse
+        s->gen_epoch = ~0ull;
+        plan.build(*s->ctx, s->refs(), rv, 0);
+        s->gen_reqs = rv;
+        s->gen_epoch = epoch;
+    }
     plan.launch((hipStream_t)stream);
     hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
     for (int g : plan.commits)

@@ -147,6 +147,25 @@ struct Context {
     struct Plan *fill_plan = nullptr;
     char *fill_states = nullptr, *fill_out = nullptr;
     size_t fill_states_cap = 0, fill_out_cap = 0;
+    // The shape fill_plan was built for.  A call of the same shape launches
+    // it again without planning: its cells, jumps and output addresses do
+    // not depend on the start window, which the planes and jump kernels
+    // read when they run.
+    struct FillShape {
+        int slot = -1, bits = 0, kind = -1;
+        size_t n = 0;
+        const void *out = nullptr;
+        const char *states = nullptr;
+        uint64_t epoch = ~0ull;
+        bool operator==(const FillShape &o) const {
+            return slot == o.slot && bits == o.bits && kind == o.kind && n == o.n && out == o.out &&
+                   states == o.states && epoch == o.epoch;
+        }
+    };
+    FillShape fill_shape;
+    // Bumped under an exclusive tune_mu whenever a planner default may
+    // change, so that a plan kept for reuse knows it is stale.
+    uint64_t tune_epoch = 0;
 
     std::map<std::pair<const void *, int>, size_t> reserve_cache;
     // Dynamic LDS that limits `fn` to `per_cu` resident workgroups per CU
@@ -467,8 +486,14 @@ struct mtgp_genset {
     } sub;
     // mtgp_genset_generate's plan, reused so that repeated calls allocate
     // nothing once their sizes have been seen (calls on one set serialise).
+    // A call whose requests equal the last call's, with no planner default
+    // changed since (gen_epoch), re-points the built plan at the
+    // generators' current windows instead of planning again: its cells,
+    // jumps and output addresses depend only on the requests.
     std::mutex gen_mu;
     std::unique_ptr<mtgp::Plan> gen_plan;
+    std::vector<mtgp_request_t> gen_reqs;
+    uint64_t gen_epoch = ~0ull;
     ~mtgp_genset() {
         gen_plan.reset();
         if (d_states)

@@ -53,3 +53,80 @@ def test_generators_of_one_set_share_the_cache(mtgp, oracle):
             got = g.fill(0, n)
             want, ref[k][1] = oracle.generate(ref[k][0], ref[k][1], 0, n)
             _same(got, want, "round %d generator %d" % (rnd, k))
+
+
+@pytest.mark.parametrize("bits,mexp", [(32, 11213), (64, 23209)])
+def test_repeated_fills_of_one_shape(mtgp, oracle, bits, mexp):
+    """Fills of one shape in a row launch the plan built for the first (no
+    planning); a changed planner knob, another generator of the set and a
+    different kind in between each still continue every stream exactly."""
+    print("repeated fills MTGP%d-%d" % (bits, mexp), flush=True)
+    Gen = mtgp.Generator32 if bits == 32 else mtgp.Generator64
+    gens = [Gen(mexp, 4, seed=71), Gen(mexp, 4, seed=72)]
+    p = oracle.params(bits, mexp, 4)
+    ws = [oracle.init_state(p, 71), oracle.init_state(p, 72)]
+    n = 1 << 21
+    buf = mtgp.DeviceBuffer(n * bits // 8)
+    wd = np.uint32 if bits == 32 else np.uint64
+    steps = [(0, 0), (0, 0), (1, 0), (0, 0), ("knob", None), (0, 0), (0, 2), (0, 0)]
+    old = mtgp.tuning_get("jump_waves_per_cu")
+    try:
+        for k, (g, kind) in enumerate(steps):
+            if g == "knob":
+                mtgp.tuning_set("jump_waves_per_cu", 12 if old != 12 else 8)
+                continue
+            gens[g].fill(kind, n, buf)
+            want, ws[g] = oracle.generate(p, ws[g], kind, n)
+            _same(buf.to_host(wd, n), want, "fill %d (generator %d, kind %d)" % (k, g, kind))
+    finally:
+        mtgp.tuning_set("jump_waves_per_cu", old)
+    for g in range(2):
+        assert (gens[g].window() == ws[g]).all()
+
+
+def test_repeated_generate_calls_of_one_shape(mtgp, oracle):
+    """mtgp_genset_generate with the same requests in a row re-points its
+    built plan at the generators' current windows; other requests, a jump
+    and a changed planner knob in between each still give every stream
+    exactly."""
+    print("repeated genset generate calls", flush=True)
+    specs = [(32, 11213, 2), (64, 23209, 3), (32, 23209, 4)]
+    seeds = [5, 6, 7]
+    gs = mtgp.GenSet(specs)
+    gs.init_seed(seeds)
+    ref = []
+    for (bits, mexp, idx), sd in zip(specs, seeds):
+        p = oracle.params(bits, mexp, idx)
+        ref.append([p, oracle.init_state(p, sd), bits])
+    counts = [3 << 19, 1 << 20, (1 << 20) + 77]
+    bufs = [mtgp.DeviceBuffer(c * b // 8) for c, (b, _, _) in zip(counts, specs)]
+    shape = [(g, (0, 2, 1)[g], counts[g], 0, bufs[g].address(0)) for g in range(3)]
+
+    def run(reqs):
+        gs.generate(reqs)
+        for g, kind, count, skip, _ in reqs:
+            p, w, bits = ref[g]
+            if skip:
+                _, w = oracle.generate(p, w, 0, skip)
+            e, w = oracle.generate(p, w, kind, count)
+            ref[g][1] = w
+            got = bufs[g].to_host(np.uint32 if bits == 32 else np.uint64, count)
+            _same(got, e, "generator %d" % g)
+
+    old = mtgp.tuning_get("jump_waves_per_cu")
+    try:
+        run(shape)
+        run(shape)
+        run([shape[1], (0, 0, 5000, 12345, bufs[0].address(0))])
+        run(shape)
+        js = mtgp.jump_hex(64, 23209, 3, 10 ** 9 + 7)
+        gs.jump(1, js)
+        ref[1][1] = oracle.jump(ref[1][0], ref[1][1], js)
+        run(shape)
+        mtgp.tuning_set("jump_waves_per_cu", 12 if old != 12 else 8)
+        run(shape)
+        run(shape)
+    finally:
+        mtgp.tuning_set("jump_waves_per_cu", old)
+    for g in range(3):
+        assert (gs.state(g) == ref[g][1]).all(), "window of generator %d" % g
