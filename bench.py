@@ -435,7 +435,8 @@ def jump_latency(M, reps=20):
         out["fast_jump_noalloc_ms"] = round((time.perf_counter() - t) * 1e3 / reps, 4)
     # mtgp32_fill, the drop-in's bulk call (window up, a plan over the GPU,
     # window back, synchronous): per-call time for a host array of 2^10 and
-    # 2^20 numbers and a device buffer of 2^20 and 2^24
+    # 2^20 numbers and a device buffer of 2^20 and 2^24, after one call of
+    # the shape (which plans; the calls timed relaunch that plan)
     fills = {}
     for n, dev in ((1 << 10, False), (1 << 20, False), (1 << 20, True), (1 << 24, True)):
         dst = M.DeviceBuffer(4 * n) if dev else np.empty(n, np.uint32)
@@ -466,7 +467,8 @@ def jump_latency(M, reps=20):
                    "their partial windows XORed by a second launch, window back, synchronous); "
                    "jump_poly = x^(3^162) mod P on the host; reference = the reference library on one core; "
                    "fill_ms = one mtgp32_fill of 2^k uint32 into a host array or a device buffer (window up, "
-                   "plan, launches, window back, synchronous); reference_cpu_genrand = the same count of "
+                   "launches of the plan built by the first call of that shape, window back, synchronous); "
+                   "reference_cpu_genrand = the same count of "
                    "mtgp32_genrand_uint32 on one core")
     return out
 
