@@ -103,6 +103,9 @@ int mtgp_event_elapsed_ms(void *start, void *stop, float *ms);
  * host memory) and leaves `mtgp` exactly where n calls of the matching
  * genrand_* would have left it (same window; the ring is rewritten with
  * idx = size - 1).  Long requests are split across the GPU by jump-ahead.
+ * A call of the same shape as the previous one on the device (parameter
+ * set, kind, n, out; no tuning change since) relaunches the plan built
+ * for it.  Calls on one device serialise.
  * Replaces a loop of mtgp32_genrand_* (mtgp32-fast.h:252-346). */
 int mtgp32_fill(mtgp32_fast_t *mtgp32, int kind, void *out, size_t n);
 int mtgp64_fill(mtgp64_fast_t *mtgp64, int kind, void *out, size_t n);
@@ -269,7 +272,9 @@ int mtgp_plan_jump_stats(const mtgp_plan_t *plan, int *jumps, int *matrix_jumps,
  * that), or -1. */
 int mtgp_plan_launch_info(const mtgp_plan_t *plan, int k, int *bits, int *cls, int *layout, int *grid,
                           int *items, char *kernel, size_t cap);
-/* plan + launch + commit + synchronize. */
+/* plan + launch + commit + synchronize.  Requests equal to the previous
+ * call's on this set (and no tuning change since) rebase the plan already
+ * built instead of planning again. */
 int mtgp_genset_generate(mtgp_genset_t *set, const mtgp_request_t *reqs, int nreq, void *stream);
 
 /* ---------------------------------------------------------------- serving
