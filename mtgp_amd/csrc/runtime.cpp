@@ -1883,7 +1883,15 @@ struct mtgp_plan {
 // the polynomial in sequence), so it is split into partial jumps over chunk
 // ranges, one wave each, whose windows a second launch XORs together (the
 // identity polynomial with JumpItem.in_parts) into `out`.
-static void device_jump(Context &c, int slot, int bits, int mexp, const void *in, void *out, const char *hex) {
+// One jump of a window by a jump string: the coefficients split over up to
+// jump_max_parts waves by chunk range, their partial windows XORed by a
+// second launch.  In the context's workspace [masks | items | queues |
+// window in | partial windows | window out]; everything the kernels read
+// goes up in one copy from the pinned staging copy of that prefix.  With
+// host_io, `in` and `out` are host windows staged through the workspace
+// (one copy each way), else device windows.
+static void device_jump(Context &c, int slot, int bits, int mexp, const void *in, void *out, const char *hex,
+                        bool host_io) {
     gf2::Poly P;
     if (!gf2::from_hex(hex, P))
         throw Error("jump string contains a non-hex character");
@@ -1898,51 +1906,58 @@ static void device_jump(Context &c, int slot, int bits, int mexp, const void *in
     const int parts = std::max(1, std::min(chunks, c.jump_max_parts));
     const int wbytes = state_words(bits, mexp) * (bits / 8);
     const size_t stride = (wbytes + 255) & ~(size_t)255;
-    // One workspace: [masks | items | 2 queue pairs | partial windows].
     const size_t o_items = (masks.size() * 8 + 255) & ~(size_t)255;
     const size_t o_queue = o_items + (((size_t)(parts + 1) * sizeof(JumpItem) + 255) & ~(size_t)255);
-    const size_t o_parts = o_queue + 256;
-    char *d = nullptr;
-    hip_check(hipMalloc((void **)&d, o_parts + (size_t)parts * stride), "hipMalloc jump workspace");
-    try {
-        std::vector<JumpItem> items((size_t)parts + 1);
-        for (int q = 0; q < parts; q++) {
-            JumpItem &ji = items[(size_t)q];
-            ji.state_in = (uint64_t)(uintptr_t)in;
-            ji.masks = (uint64_t)(uintptr_t)d;
-            ji.states_out = (uint64_t)(uintptr_t)(d + o_parts + (size_t)q * stride);
-            ji.nb = 1;
-            ji.c0 = (int32_t)((long)chunks * q / parts);
-            ji.c1 = (int32_t)((long)chunks * (q + 1) / parts);
-            ji.param = slot;
-            ji.state_bytes = wbytes;
-        }
-        JumpItem &cj = items[(size_t)parts];
-        cj.state_in = (uint64_t)(uintptr_t)(d + o_parts);
-        cj.in_parts = parts;
-        cj.in_part_bytes = stride;
-        cj.masks = (uint64_t)(uintptr_t)(d + id_off * 8);
-        cj.states_out = (uint64_t)(uintptr_t)out;
-        cj.nb = 1;
-        cj.c0 = 0;
-        cj.c1 = 1;
-        cj.param = slot;
-        cj.state_bytes = wbytes;
-        hip_check(hipMemcpy(d, masks.data(), masks.size() * 8, hipMemcpyHostToDevice), "hipMemcpy");
-        hip_check(hipMemcpy(d + o_items, items.data(), items.size() * sizeof(JumpItem), hipMemcpyHostToDevice),
-                  "hipMemcpy");
-        hip_check(hipMemset(d + o_queue, 0, 4 * sizeof(uint32_t)), "hipMemset");
-        const int cls = c.class_of(slot);
-        const JumpItem *d_items = reinterpret_cast<const JumpItem *>(d + o_items);
-        uint32_t *d_q = reinterpret_cast<uint32_t *>(d + o_queue);
-        hip_check(launch_jumps(bits, cls, d_items, parts, c.d_params, d_q, parts, 0, 0), "jump kernel launch");
-        hip_check(launch_jumps(bits, cls, d_items + parts, 1, c.d_params, d_q + 2, 1, 0, 0), "jump kernel launch");
-        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    } catch (...) {
-        (void)hipFree(d);
-        throw;
+    const size_t o_in = o_queue + 256;
+    const size_t o_parts = o_in + stride;
+    const size_t o_out = o_parts + (size_t)parts * stride;
+    const size_t up = host_io ? o_in + (size_t)wbytes : o_in;  // bytes uploaded
+    std::lock_guard<std::mutex> lk(c.jump_mu);
+    grow_device(c.jump_d, c.jump_dcap, o_out + stride, "hipMalloc jump workspace");
+    grow_pinned(c.jump_h, c.jump_hcap, o_out + stride);
+    char *d = c.jump_d, *h = c.jump_h;
+    const char *win_in = host_io ? d + o_in : static_cast<const char *>(in);
+    char *win_out = host_io ? d + o_out : static_cast<char *>(out);
+    memset(h, 0, o_in);
+    memcpy(h, masks.data(), masks.size() * 8);
+    JumpItem *items = reinterpret_cast<JumpItem *>(h + o_items);
+    for (int q = 0; q < parts; q++) {
+        JumpItem &ji = items[q];
+        ji.state_in = (uint64_t)(uintptr_t)win_in;
+        ji.masks = (uint64_t)(uintptr_t)d;
+        ji.states_out = (uint64_t)(uintptr_t)(d + o_parts + (size_t)q * stride);
+        ji.nb = 1;
+        ji.c0 = (int32_t)((long)chunks * q / parts);
+        ji.c1 = (int32_t)((long)chunks * (q + 1) / parts);
+        ji.param = slot;
+        ji.state_bytes = wbytes;
     }
-    (void)hipFree(d);
+    JumpItem &cj = items[parts];
+    cj.state_in = (uint64_t)(uintptr_t)(d + o_parts);
+    cj.in_parts = parts;
+    cj.in_part_bytes = stride;
+    cj.masks = (uint64_t)(uintptr_t)(d + id_off * 8);
+    cj.states_out = (uint64_t)(uintptr_t)win_out;
+    cj.nb = 1;
+    cj.c0 = 0;
+    cj.c1 = 1;
+    cj.param = slot;
+    cj.state_bytes = wbytes;
+    if (host_io)
+        memcpy(h + o_in, in, (size_t)wbytes);
+    // the queue pairs go up as zeros with the rest
+    hip_check(hipMemcpyAsync(d, h, up, hipMemcpyHostToDevice, nullptr), "hipMemcpyAsync jump workspace");
+    const int cls = c.class_of(slot);
+    const JumpItem *d_items = reinterpret_cast<const JumpItem *>(d + o_items);
+    uint32_t *d_q = reinterpret_cast<uint32_t *>(d + o_queue);
+    hip_check(launch_jumps(bits, cls, d_items, parts, c.d_params, d_q, parts, 0, nullptr), "jump kernel launch");
+    hip_check(launch_jumps(bits, cls, d_items + parts, 1, c.d_params, d_q + 2, 1, 0, nullptr), "jump kernel launch");
+    if (host_io)
+        hip_check(hipMemcpyAsync(h + o_out, d + o_out, (size_t)wbytes, hipMemcpyDeviceToHost, nullptr),
+                  "hipMemcpyAsync jumped window");
+    hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+    if (host_io)
+        memcpy(out, h + o_out, (size_t)wbytes);
 }
 
 template <class Gen, class Word>
@@ -2025,18 +2040,7 @@ static void fast_jump_impl(Gen *g, const char *hex, int bits) {
     std::vector<Word> w((size_t)N);
     for (int j = 0; j < N; j++)
         w[(size_t)j] = g->status->array[(g->status->idx - N + 1 + j + g->status->large_size) & g->status->large_mask];
-    char *d = nullptr;
-    hip_check(hipMalloc((void **)&d, 2 * (size_t)N * sizeof(Word)), "hipMalloc");
-    try {
-        hip_check(hipMemcpy(d, w.data(), (size_t)N * sizeof(Word), hipMemcpyHostToDevice), "hipMemcpy");
-        device_jump(c, slot, bits, g->params.mexp, d, d + (size_t)N * sizeof(Word), hex);
-        hip_check(hipMemcpy(w.data(), d + (size_t)N * sizeof(Word), (size_t)N * sizeof(Word), hipMemcpyDeviceToHost),
-                  "hipMemcpy");
-    } catch (...) {
-        (void)hipFree(d);
-        throw;
-    }
-    (void)hipFree(d);
+    device_jump(c, slot, bits, g->params.mexp, w.data(), w.data(), hex, true);
     memcpy(g->status->array, w.data(), (size_t)N * sizeof(Word));
     g->status->idx = N - 1;
 }
@@ -2722,7 +2726,7 @@ int mtgp_genset_jump(mtgp_genset_t *s, int g, const char *hex) {
         throw Error("generator index out of range");
     const int cur = s->cur[(size_t)g];
     device_jump(*s->ctx, s->slot[(size_t)g], s->bits[(size_t)g], s->mexp[(size_t)g], s->state(g, cur),
-                s->state(g, cur ^ 1), hex);
+                s->state(g, cur ^ 1), hex, false);
     s->set_cur(g, cur ^ 1);
     return 0;
     API_END(-1)

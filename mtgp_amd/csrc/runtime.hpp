@@ -163,6 +163,12 @@ struct Context {
         }
     };
     FillShape fill_shape;
+    // The single-jump path (mtgp*_fast_jump, mtgp_genset_jump) keeps one
+    // grow-only device workspace and its pinned staging copy; calls on one
+    // device serialise on jump_mu.  Kept until the process ends.
+    std::mutex jump_mu;
+    char *jump_d = nullptr, *jump_h = nullptr;
+    size_t jump_dcap = 0, jump_hcap = 0;
     // Bumped under an exclusive tune_mu whenever a planner default may
     // change, so that a plan kept for reuse knows it is stale.
     uint64_t tune_epoch = 0;
