@@ -1995,7 +1995,15 @@ static int fill_impl(Gen *g, int kind, void *out, size_t n, int bits) {
         grow_device(c.fill_out, c.fill_out_cap, n * eb, "hipMalloc fill output");
         d_out = c.fill_out;
     }
-    hip_check(hipMemcpy(d_states, w.data(), (size_t)N * sizeof(Word), hipMemcpyHostToDevice), "hipMemcpy");
+    // Windows (and host outputs up to kFillStage bytes) go through pinned
+    // staging: one asynchronous copy each way and one synchronisation.
+    constexpr size_t kFillStage = (size_t)64 << 20;
+    const size_t wb = (size_t)N * sizeof(Word);
+    const bool stage_out = n && !dev_out && n * eb <= kFillStage;
+    grow_pinned(c.fill_h, c.fill_hcap, 2 * wb + (stage_out ? n * eb : 0));
+    char *h_in = c.fill_h, *h_win = c.fill_h + wb, *h_out = c.fill_h + 2 * wb;
+    memcpy(h_in, w.data(), wb);
+    hip_check(hipMemcpyAsync(d_states, h_in, wb, hipMemcpyHostToDevice, nullptr), "hipMemcpyAsync window");
     GenRef r;
     r.bits = bits;
     r.mexp = g->params.mexp;
@@ -2021,11 +2029,15 @@ static int fill_impl(Gen *g, int kind, void *out, size_t n, int bits) {
         c.fill_shape = shape;
     }
     plan.launch(0);
-    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    hip_check(hipMemcpy(w.data(), r.state_out, (size_t)N * sizeof(Word), hipMemcpyDeviceToHost), "hipMemcpy");
-    if (n && !dev_out)
+    hip_check(hipMemcpyAsync(h_win, r.state_out, wb, hipMemcpyDeviceToHost, nullptr), "hipMemcpyAsync window");
+    if (stage_out)
+        hip_check(hipMemcpyAsync(h_out, d_out, n * eb, hipMemcpyDeviceToHost, nullptr), "hipMemcpyAsync output");
+    hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+    if (stage_out)
+        memcpy(out, h_out, n * eb);
+    else if (n && !dev_out)
         hip_check(hipMemcpy(out, d_out, n * eb, hipMemcpyDeviceToHost), "hipMemcpy");
-    memcpy(g->status->array, w.data(), (size_t)N * sizeof(Word));
+    memcpy(g->status->array, h_win, wb);
     g->status->idx = N - 1;
     return 0;
 }

@@ -147,6 +147,10 @@ struct Context {
     struct Plan *fill_plan = nullptr;
     char *fill_states = nullptr, *fill_out = nullptr;
     size_t fill_states_cap = 0, fill_out_cap = 0;
+    // Pinned staging of the windows and of host outputs up to 64 MiB (grows
+    // to the largest such call and is kept until the process ends).
+    char *fill_h = nullptr;
+    size_t fill_hcap = 0;
     // The shape fill_plan was built for.  A call of the same shape launches
     // it again without planning: its cells, jumps and output addresses do
     // not depend on the start window, which the planes and jump kernels
