@@ -1,9 +1,11 @@
-"""Repeated drop-in fills (mtgp32_fill / mtgp64_fill): one reused plan per
-device and jump polynomials from the process-wide cache of uniform grids.
-Streams continue bit-exactly across calls of changing sizes, generators of
-one parameter set share a cache, other sets and word sizes do not get its
-polynomials, and host and device destinations agree -- all against the
-oracle."""
+"""Repeated drop-in fills (mtgp32_fill / mtgp64_fill) and genset generate
+calls: one reused plan per device (per set), relaunched without planning
+for calls of the same shape, and jump polynomials from the process-wide
+cache of uniform grids.  Streams continue bit-exactly across calls of
+changing sizes, generators of one parameter set share a cache, other sets
+and word sizes do not get its polynomials, host and device destinations
+agree, and other requests, jumps and knob changes between calls of one
+shape leave every stream exact -- all against the oracle."""
 import numpy as np
 import pytest
 
